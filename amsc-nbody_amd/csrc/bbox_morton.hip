@@ -1,0 +1,397 @@
+// bbox_morton.hip -- global bounding box, Morton encoding and the
+// Morton-order particle gather.
+//
+// Bounding box: one pass, FP64 min/max per lane, wave64 shuffles, LDS across
+// the 4 waves of a block, per-block partials, and the last block (arrival
+// counter) finishes the box on the device with the reference's arithmetic
+// (globalbounding.cu:122-136: centre = (min+max)*0.5, side = 1.1*max range,
+// side 0 -> 1).  The kick-drift variant applies the first half kick and the
+// drift of the leapfrog step (integrator.cu:51-83) in the same pass, so the
+// new positions are reduced while they are in registers.
+//
+// Encode: normalise -> quantise -> interleave fused in one kernel
+// (morton.cu:98-139 run as 7 launches in the reference), writing the key,
+// the identity index and the digit histograms of every radix-sort pass.
+#include "device_utils.hpp"
+#include "nbh_internal.hpp"
+
+#include <cmath>
+
+namespace nbh {
+
+namespace {
+
+constexpr int kBoxThreads = 256;
+constexpr int kBoxMaxBlocks = 1024;
+
+struct Extent {
+    double lo[3];
+    double hi[3];
+    bool bad;
+};
+
+__device__ __forceinline__ void block_reduce_store(Extent e, double* partials)
+{
+    __shared__ double s_red[4][7];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    double v[7];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        v[a] = wave_min(e.lo[a]);
+        v[3 + a] = wave_max(e.hi[a]);
+    }
+    v[6] = (__ballot(e.bad) != 0ull) ? 1.0 : 0.0;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            s_red[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            r[k] = s_red[0][k];
+        for (int w = 1; w < 4; ++w) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                r[a] = fmin(r[a], s_red[w][a]);
+                r[3 + a] = fmax(r[3 + a], s_red[w][3 + a]);
+            }
+            r[6] = fmax(r[6], s_red[w][6]);
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            partials[blockIdx.x * 8 + k] = r[k];
+    }
+}
+
+// The last block to arrive reduces all partials and writes the box.
+__device__ __forceinline__ void finish_box(const double* partials,
+                                           unsigned* counter, double* box,
+                                           int* err)
+{
+    __shared__ bool s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned done = atomicAdd(counter, 1u);
+        s_last = (done == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last)
+        return;
+    __threadfence();
+    const double inf = __builtin_inf();
+    Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false};
+    double bad = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+        const double* p = partials + b * 8;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            e.lo[a] = fmin(e.lo[a], p[a]);
+            e.hi[a] = fmax(e.hi[a], p[3 + a]);
+        }
+        bad = fmax(bad, p[6]);
+    }
+    e.bad = bad != 0.0;
+    __shared__ double s_red[4][7];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    double v[7];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        v[a] = wave_min(e.lo[a]);
+        v[3 + a] = wave_max(e.hi[a]);
+    }
+    v[6] = (__ballot(e.bad) != 0ull) ? 1.0 : 0.0;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            s_red[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            r[k] = s_red[0][k];
+        for (int w = 1; w < 4; ++w) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                r[a] = fmin(r[a], s_red[w][a]);
+                r[3 + a] = fmax(r[3 + a], s_red[w][3 + a]);
+            }
+            r[6] = fmax(r[6], s_red[w][6]);
+        }
+        // globalbounding.cu:122-136
+        const double cx = (r[0] + r[3]) * 0.5;
+        const double cy = (r[1] + r[4]) * 0.5;
+        const double cz = (r[2] + r[5]) * 0.5;
+        const double rx = r[3] - r[0];
+        const double ry = r[4] - r[1];
+        const double rz = r[5] - r[2];
+        double m = rx;
+        if (m < ry)
+            m = ry;
+        if (m < rz)
+            m = rz;
+        double side = 1.1 * m;
+        if (side == 0.0)
+            side = 1.0;
+        box[0] = cx;
+        box[1] = cy;
+        box[2] = cz;
+        box[3] = side;
+        if (r[6] != 0.0)
+            atomicCAS(err, 0, 1);
+        *counter = 0u; // ready for the next launch
+    }
+}
+
+__global__ __launch_bounds__(kBoxThreads) void bbox_kernel(
+    const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, size_t n, double* __restrict__ partials,
+    unsigned* counter, double* box, int* err)
+{
+    const double inf = __builtin_inf();
+    Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false};
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         i < n; i += stride) {
+        const double px = x[i], py = y[i], pz = z[i];
+        e.bad = e.bad || !(isfinite(px) && isfinite(py) && isfinite(pz));
+        e.lo[0] = fmin(e.lo[0], px);
+        e.lo[1] = fmin(e.lo[1], py);
+        e.lo[2] = fmin(e.lo[2], pz);
+        e.hi[0] = fmax(e.hi[0], px);
+        e.hi[1] = fmax(e.hi[1], py);
+        e.hi[2] = fmax(e.hi[2], pz);
+    }
+    block_reduce_store(e, partials);
+    finish_box(partials, counter, box, err);
+}
+
+// First half kick + drift (integrator.cu:51-83: v += (dt/2) a; x += dt v,
+// contracted to fma) fused with the bounding-box reduction of the new
+// positions.
+__global__ __launch_bounds__(kBoxThreads) void kick_drift_bbox_kernel(
+    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, size_t n, double dt,
+    double* __restrict__ partials, unsigned* counter, double* box, int* err)
+{
+    const double half = 0.5 * dt;
+    const double inf = __builtin_inf();
+    Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false};
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         i < n; i += stride) {
+        const double wx = fma(half, ax[i], vx[i]);
+        const double wy = fma(half, ay[i], vy[i]);
+        const double wz = fma(half, az[i], vz[i]);
+        const double px = fma(dt, wx, x[i]);
+        const double py = fma(dt, wy, y[i]);
+        const double pz = fma(dt, wz, z[i]);
+        vx[i] = wx;
+        vy[i] = wy;
+        vz[i] = wz;
+        x[i] = px;
+        y[i] = py;
+        z[i] = pz;
+        e.bad = e.bad || !(isfinite(px) && isfinite(py) && isfinite(pz));
+        e.lo[0] = fmin(e.lo[0], px);
+        e.lo[1] = fmin(e.lo[1], py);
+        e.lo[2] = fmin(e.lo[2], pz);
+        e.hi[0] = fmax(e.hi[0], px);
+        e.hi[1] = fmax(e.hi[1], py);
+        e.hi[2] = fmax(e.hi[2], pz);
+    }
+    block_reduce_store(e, partials);
+    finish_box(partials, counter, box, err);
+}
+
+unsigned box_blocks(size_t n)
+{
+    unsigned b = blocks_for(n, kBoxThreads * 4);
+    return b > kBoxMaxBlocks ? kBoxMaxBlocks : b;
+}
+
+// ------------------------------------------------------------- Morton
+// 10-bit spread with the reference's masks (morton.cu:119-128).
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    uint32_t x = v & 0x000003FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// 21-bit spread: bit k -> bit 3k, by halving shift distances.
+__device__ __forceinline__ uint64_t spread21(uint32_t v)
+{
+    uint64_t x = v & 0x1FFFFFull;
+    x = (x | (x << 32)) & 0x001F00000000FFFFull;
+    x = (x | (x << 16)) & 0x001F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+template <int BITS>
+__device__ __forceinline__ typename Layout<BITS>::Key morton_key(uint32_t qx,
+                                                                 uint32_t qy,
+                                                                 uint32_t qz)
+{
+    if constexpr (BITS == 10)
+        return (spread10(qx) << 2) | (spread10(qy) << 1) | spread10(qz);
+    else
+        return (spread21(qx) << 2) | (spread21(qy) << 1) | spread21(qz);
+}
+
+// morton.cu:98-117: (a - (c - s/2)) / s, * 2^bits, clamp, truncate.
+template <int BITS>
+__device__ __forceinline__ uint32_t quantise(double a, double origin,
+                                             double side)
+{
+    constexpr double grid = static_cast<double>(1u << BITS);
+    constexpr double maxq = static_cast<double>((1u << BITS) - 1u);
+    const double normalised = (a - origin) / side;
+    const double scaled = normalised * grid;
+    const double clamped = fmin(fmax(scaled, 0.0), maxq);
+    return static_cast<uint32_t>(clamped);
+}
+
+template <int BITS>
+__global__ __launch_bounds__(256) void encode_kernel(
+    const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, size_t n, const double* __restrict__ box,
+    typename Layout<BITS>::Key* __restrict__ keys, uint32_t* __restrict__ idx,
+    uint32_t* __restrict__ hist)
+{
+    using Key = typename Layout<BITS>::Key;
+    constexpr int P = Layout<BITS>::sort_passes;
+    __shared__ uint32_t s_hist[P * 256];
+    for (int j = threadIdx.x; j < P * 256; j += blockDim.x)
+        s_hist[j] = 0;
+    __syncthreads();
+    const double side = box[3];
+    const double ox = box[0] - side / 2.0;
+    const double oy = box[1] - side / 2.0;
+    const double oz = box[2] - side / 2.0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         i < n; i += stride) {
+        const uint32_t qx = quantise<BITS>(x[i], ox, side);
+        const uint32_t qy = quantise<BITS>(y[i], oy, side);
+        const uint32_t qz = quantise<BITS>(z[i], oz, side);
+        const Key k = morton_key<BITS>(qx, qy, qz);
+        keys[i] = k;
+        idx[i] = static_cast<uint32_t>(i);
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+            atomicAdd(&s_hist[p * 256 + static_cast<uint32_t>((k >> (8 * p)) & 0xFFu)], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < P * 256; j += blockDim.x) {
+        const uint32_t c = s_hist[j];
+        if (c)
+            atomicAdd(&hist[j], c);
+    }
+}
+
+// Morton-order gather (simulation.cu:33-57) into packed FP64 records
+// {x, y, z, m}, plus the fixed-point records of the mixed traversal:
+// coordinates on a 2^31 grid over the box, and G m / unit^2 in FP32 where
+// unit = side / 2^31 (so that a = sum (G m / unit^2) d / r^3 with d, r in
+// grid units is the acceleration in physical units).
+__global__ __launch_bounds__(256) void gather_kernel(
+    const uint32_t* __restrict__ idx, const double* __restrict__ m,
+    const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, size_t n, double4* __restrict__ rec64,
+    int4* __restrict__ rec32, const double* __restrict__ box, double G)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t p = idx[i];
+    const double px = x[p], py = y[p], pz = z[p], pm = m[p];
+    rec64[i] = make_double4(px, py, pz, pm);
+    if (rec32) {
+        const double side = box[3];
+        const double scale = 2147483648.0 / side;
+        const double ox = box[0] - side / 2.0;
+        const double oy = box[1] - side / 2.0;
+        const double oz = box[2] - side / 2.0;
+        const double lim = 2147483647.0;
+        const int qx = static_cast<int>(fmin(fmax(rint((px - ox) * scale), 0.0), lim));
+        const int qy = static_cast<int>(fmin(fmax(rint((py - oy) * scale), 0.0), lim));
+        const int qz = static_cast<int>(fmin(fmax(rint((pz - oz) * scale), 0.0), lim));
+        const float gm = static_cast<float>(G * pm * scale * scale);
+        rec32[i] = make_int4(qx, qy, qz, __float_as_int(gm));
+    }
+}
+
+} // namespace
+
+void launch_bbox(nbh_context* ctx, const double* x, const double* y,
+                 const double* z, size_t n, double* d_box, double* partials,
+                 unsigned* counter, int* err)
+{
+    bbox_kernel<<<box_blocks(n), kBoxThreads, 0, ctx->stream>>>(
+        x, y, z, n, partials, counter, d_box, err);
+    NBH_HIP(hipGetLastError());
+}
+
+void launch_kick_drift_bbox(hipStream_t s, double* x, double* y, double* z,
+                            double* vx, double* vy, double* vz,
+                            const double* ax, const double* ay,
+                            const double* az, size_t n, double dt,
+                            double* d_box, double* partials,
+                            unsigned* counter, int* err)
+{
+    kick_drift_bbox_kernel<<<box_blocks(n), kBoxThreads, 0, s>>>(
+        x, y, z, vx, vy, vz, ax, ay, az, n, dt, partials, counter, d_box,
+        err);
+    NBH_HIP(hipGetLastError());
+}
+
+size_t bbox_partial_doubles() { return static_cast<size_t>(kBoxMaxBlocks) * 8; }
+
+template <int BITS>
+void launch_encode(hipStream_t s, const double* x, const double* y,
+                   const double* z, size_t n, const double* d_box,
+                   typename Layout<BITS>::Key* keys, uint32_t* idx,
+                   uint32_t* hist)
+{
+    unsigned blocks = blocks_for(n, 256 * 8);
+    if (blocks > 2048)
+        blocks = 2048;
+    encode_kernel<BITS><<<blocks, 256, 0, s>>>(x, y, z, n, d_box, keys, idx,
+                                               hist);
+    NBH_HIP(hipGetLastError());
+}
+
+template void launch_encode<10>(hipStream_t, const double*, const double*,
+                                const double*, size_t, const double*,
+                                uint32_t*, uint32_t*, uint32_t*);
+template void launch_encode<21>(hipStream_t, const double*, const double*,
+                                const double*, size_t, const double*,
+                                uint64_t*, uint32_t*, uint32_t*);
+
+void launch_gather(hipStream_t s, const uint32_t* idx, const double* m,
+                   const double* x, const double* y, const double* z,
+                   size_t n, double4* rec64, int4* rec32, const double* box,
+                   double G)
+{
+    gather_kernel<<<blocks_for(n, 256), 256, 0, s>>>(idx, m, x, y, z, n, rec64,
+                                                     rec32, box, G);
+    NBH_HIP(hipGetLastError());
+}
+
+} // namespace nbh

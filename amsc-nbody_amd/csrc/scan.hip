@@ -1,0 +1,331 @@
+// scan.hip -- single-pass decoupled look-back scans.
+//
+// Leaf groups (replaces cub::DeviceRunLengthEncode::Encode +
+// cub::DeviceScan::ExclusiveSum, morton_leaf_groups.cu:58-103): one kernel
+// scans the pair (head flag, position of the latest head) with (+, max);
+// each element then knows its group id and, at a run's tail, the run length,
+// so unique keys, first particles and counts are written in the same pass
+// and the group count stays on the device.
+//
+// Radix -> octree plan (replaces computeRadixToOctreePlanKernel + its
+// ExclusiveSum, radix_to_octree.cu:54-145,174-244): each element computes the
+// octree level and edge node count of one radix node and the same kernel
+// scans the counts; the octree node total stays on the device.
+//
+// Tiles take tickets in launch order; a tile publishes its aggregate, looks
+// back over earlier tiles (64-bit status words: 2 flag bits + 62 payload
+// bits, agent-scope atomics) and publishes its inclusive prefix.
+#include "device_utils.hpp"
+#include "nbh_internal.hpp"
+
+namespace nbh {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr unsigned long long kAgg = 1ull << 62;
+constexpr unsigned long long kInc = 2ull << 62;
+constexpr unsigned long long kPayload = (1ull << 62) - 1ull;
+
+struct SumMax {
+    int a; // number of heads so far
+    int b; // position of the latest head so far
+};
+
+__device__ __forceinline__ SumMax combine(SumMax x, SumMax y)
+{
+    return {x.a + y.a, x.b > y.b ? x.b : y.b};
+}
+__device__ __forceinline__ long long combine(long long x, long long y)
+{
+    return x + y;
+}
+__device__ __forceinline__ SumMax shfl_up(SumMax v, int o)
+{
+    return {__shfl_up(v.a, o), __shfl_up(v.b, o)};
+}
+__device__ __forceinline__ long long shfl_up(long long v, int o)
+{
+    return __shfl_up(v, o);
+}
+__device__ __forceinline__ unsigned long long pack(SumMax v)
+{
+    return (static_cast<unsigned long long>(v.a) << 31) |
+           static_cast<unsigned long long>(v.b);
+}
+__device__ __forceinline__ unsigned long long pack(long long v)
+{
+    return static_cast<unsigned long long>(v) & kPayload;
+}
+__device__ __forceinline__ void unpack(unsigned long long w, SumMax& v)
+{
+    v.a = static_cast<int>((w >> 31) & 0x7FFFFFFFull);
+    v.b = static_cast<int>(w & 0x7FFFFFFFull);
+}
+__device__ __forceinline__ void unpack(unsigned long long w, long long& v)
+{
+    v = static_cast<long long>(w & kPayload);
+}
+
+// Scan of one tile whose thread-inclusive values are in v[0..kScanItems).
+// On return v[] holds the global inclusive values.  `identity` must be the
+// op's identity.  Thread 0 performs the look-back.
+template <typename T>
+__device__ void tile_scan(T (&v)[kScanItems], T identity, uint32_t tile,
+                          unsigned long long* status)
+{
+    __shared__ T s_wave[4];
+    __shared__ T s_prefix;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    // thread-local inclusive
+#pragma unroll
+    for (int k = 1; k < kScanItems; ++k)
+        v[k] = combine(v[k - 1], v[k]);
+    const T agg = v[kScanItems - 1];
+    // wave inclusive over thread aggregates
+    T x = agg;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T y = shfl_up(x, o);
+        if (lane >= o)
+            x = combine(y, x);
+    }
+    if (lane == 63)
+        s_wave[wave] = x;
+    __syncthreads();
+    T wave_prefix = identity;
+    T tile_total = identity;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        if (w < wave)
+            wave_prefix = combine(wave_prefix, s_wave[w]);
+        tile_total = combine(tile_total, s_wave[w]);
+    }
+    // exclusive prefix of this thread inside the tile
+    T excl = shfl_up(x, 1);
+    if (lane == 0)
+        excl = identity;
+    excl = combine(wave_prefix, excl);
+    if (threadIdx.x == 0) {
+        T prefix = identity;
+        if (tile == 0) {
+            store_agent(status, kInc | pack(tile_total));
+        } else {
+            store_agent(status + tile, kAgg | pack(tile_total));
+            int j = static_cast<int>(tile) - 1;
+            while (true) {
+                const unsigned long long w = load_agent(status + j);
+                const unsigned long long flag = w & ~kPayload;
+                if (flag == 0ull) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                T val;
+                unpack(w, val);
+                prefix = combine(val, prefix);
+                if (flag == kInc)
+                    break;
+                --j;
+            }
+            store_agent(status + tile, kInc | pack(combine(prefix, tile_total)));
+        }
+        s_prefix = prefix;
+    }
+    __syncthreads();
+    const T before = combine(s_prefix, excl);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        v[k] = combine(before, v[k]);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kScanThreads) void leaf_groups_kernel(
+    const K* __restrict__ keys, int n, K* __restrict__ unique,
+    int* __restrict__ first, int* __restrict__ count, int* __restrict__ ngroups,
+    unsigned long long* status, unsigned* ticket)
+{
+    __shared__ uint32_t s_tile;
+    if (threadIdx.x == 0)
+        s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int base = static_cast<int>(tile) * kScanTile + threadIdx.x * kScanItems;
+    K k[kScanItems + 2];
+    // k[0] = key[base-1], k[1..ITEMS] = keys, k[ITEMS+1] = key[base+ITEMS]
+#pragma unroll
+    for (int j = 0; j < kScanItems + 2; ++j) {
+        const int i = base - 1 + j;
+        k[j] = (i >= 0 && i < n) ? keys[i] : K(0);
+    }
+    SumMax v[kScanItems];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const int i = base + j;
+        const bool head = i < n && (i == 0 || k[j + 1] != k[j]);
+        v[j] = {head ? 1 : 0, head ? i : 0};
+    }
+    tile_scan<SumMax>(v, SumMax{0, 0}, tile, status);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const int i = base + j;
+        if (i >= n)
+            break;
+        const int g = v[j].a - 1;
+        const bool head = (i == 0 || k[j + 1] != k[j]);
+        const bool tail = (i == n - 1 || k[j + 1] != k[j + 2]);
+        if (head) {
+            unique[g] = k[j + 1];
+            first[g] = i;
+        }
+        if (tail)
+            count[g] = i + 1 - v[j].b;
+        if (i == n - 1)
+            *ngroups = v[j].a;
+    }
+}
+
+// radix_to_octree.cu:54-68 (spatial prefix, level) and :70-145 (per-node
+// plan with error codes), fused with the exclusive scan of edge counts.
+template <int BITS>
+__global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(
+    const int* __restrict__ d_nleaves, const int* __restrict__ tparent,
+    const int* __restrict__ prefix_len,
+    const typename Layout<BITS>::Key* __restrict__ unique,
+    int* __restrict__ level, int* __restrict__ count, int* __restrict__ offset,
+    int* __restrict__ d_noctree, int* err, unsigned long long* status,
+    unsigned* ticket)
+{
+    using L = Layout<BITS>;
+    __shared__ uint32_t s_tile;
+    if (threadIdx.x == 0)
+        s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int nl = *d_nleaves;
+    const int R = 2 * nl - 1;
+    const uint32_t tile = s_tile;
+    if (static_cast<long long>(tile) * kScanTile >= R)
+        return; // beyond the last tile: nobody looks back at it
+    const int base = static_cast<int>(tile) * kScanTile + threadIdx.x * kScanItems;
+    auto level_of = [&](int node) {
+        const int sp = (node < nl) ? L::spatial_bits
+                                   : prefix_len[node - nl] - L::padding_bits;
+        return sp / 3;
+    };
+    long long v[kScanItems];
+    int lv[kScanItems];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const int node = base + j;
+        v[j] = 0;
+        lv[j] = 0;
+        if (node >= R)
+            continue;
+        bool ok = true;
+        if (node < nl) {
+            if ((unique[node] & ~L::spatial_mask) != 0) {
+                raise_error(err, 1);
+                ok = false;
+            }
+        } else {
+            const int sp = prefix_len[node - nl] - L::padding_bits;
+            if (sp < 0 || sp >= L::spatial_bits) {
+                raise_error(err, 4);
+                ok = false;
+            }
+        }
+        int parent_level = 0;
+        if (ok) {
+            const int p = tparent[node];
+            if (p != -1) {
+                if (p < nl || p >= R) {
+                    raise_error(err, 2);
+                    ok = false;
+                } else {
+                    parent_level = level_of(p);
+                }
+            }
+        }
+        if (ok) {
+            const int l = (node < nl) ? parent_level + 1 : level_of(node);
+            if (l < 0 || l > L::bits || l < parent_level) {
+                raise_error(err, 3);
+            } else {
+                lv[j] = l;
+                v[j] = l - parent_level;
+            }
+        }
+    }
+    long long c[kScanItems];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+        c[j] = v[j];
+    tile_scan<long long>(v, 0ll, tile, status);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const int node = base + j;
+        if (node >= R)
+            break;
+        level[node] = lv[j];
+        count[node] = static_cast<int>(c[j]);
+        const long long ex = v[j] - c[j];
+        offset[node] = static_cast<int>(ex);
+        if (node == R - 1) {
+            const long long total = v[j];
+            *d_noctree = (total >= 0x7FFFFFFFll) ? -1 : 1 + static_cast<int>(total);
+        }
+    }
+}
+
+} // namespace
+
+size_t scan_status_words(size_t n)
+{
+    return (n + kScanTile - 1) / kScanTile + 64;
+}
+
+// status: scan_status_words(n) zeroed 64-bit words; the ticket lives in the
+// last of them.
+template <typename K>
+void launch_leaf_groups(hipStream_t s, const K* keys, int n, K* unique,
+                        int* first, int* count, int* d_ngroups,
+                        unsigned long long* status)
+{
+    const size_t tiles = (static_cast<size_t>(n) + kScanTile - 1) / kScanTile;
+    unsigned* ticket = reinterpret_cast<unsigned*>(status + tiles + 32);
+    leaf_groups_kernel<K><<<static_cast<unsigned>(tiles), kScanThreads, 0, s>>>(
+        keys, n, unique, first, count, d_ngroups, status, ticket);
+    NBH_HIP(hipGetLastError());
+}
+
+// status: scan_status_words(2*max_leaves) zeroed words.
+template <int BITS>
+void launch_plan_scan(hipStream_t s, const int* d_nleaves, int max_leaves,
+                      const int* tparent, const int* prefix_len,
+                      const typename Layout<BITS>::Key* unique, int* level,
+                      int* count, int* offset, int* d_noctree, int* err,
+                      unsigned long long* status)
+{
+    const size_t rmax = 2 * static_cast<size_t>(max_leaves);
+    const size_t tiles = (rmax + kScanTile - 1) / kScanTile;
+    unsigned* ticket = reinterpret_cast<unsigned*>(status + tiles + 32);
+    plan_scan_kernel<BITS><<<static_cast<unsigned>(tiles), kScanThreads, 0, s>>>(
+        d_nleaves, tparent, prefix_len, unique, level, count, offset,
+        d_noctree, err, status, ticket);
+    NBH_HIP(hipGetLastError());
+}
+
+template void launch_leaf_groups<uint32_t>(hipStream_t, const uint32_t*, int, uint32_t*,
+                                           int*, int*, int*, unsigned long long*);
+template void launch_leaf_groups<uint64_t>(hipStream_t, const uint64_t*, int, uint64_t*,
+                                           int*, int*, int*, unsigned long long*);
+template void launch_plan_scan<10>(hipStream_t, const int*, int, const int*, const int*,
+                                   const uint32_t*, int*, int*, int*, int*, int*,
+                                   unsigned long long*);
+template void launch_plan_scan<21>(hipStream_t, const int*, int, const int*, const int*,
+                                   const uint64_t*, int*, int*, int*, int*, int*,
+                                   unsigned long long*);
+
+} // namespace nbh

@@ -1,0 +1,689 @@
+// simulation.hip -- the Simulation object: the whole Barnes-Hut leapfrog
+// step on one stream with two host synchronisations per step.
+//
+// Replaces Simulation (src/simulation.cpp:70-225) and its device state
+// (gpu/cuda_kernels/simulation.cu:14-142).  The reference reallocates every
+// staged structure and synchronises about twenty times per step; here every
+// buffer is sized once for N (the octree grows on demand), group and node
+// counts stay on the device, and the host reads them once, after the
+// radix-to-octree plan, to size the octree launches.  The second
+// synchronisation ends the step and reports any device error.
+//
+// Step order (src/simulation.cpp:100-119): kick-drift with the current
+// accelerations (fused with the bounding box), rebuild (Morton keys, sort,
+// groups, Karras, plan, octree, mass/COM, walk layout), forces into the next
+// buffer, closing kick, swap.
+#include "nbh_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace nbh {
+
+const char* traversal_message(int code);
+const char* stage_message(int word, int code);
+
+template <int BITS>
+void launch_climb(hipStream_t s, const nbh_octree_view& o, const double4* rec,
+                  double theta, const double* box, int* size, int* local,
+                  double* crit, int* err);
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        reset();
+        if (count == 0)
+            count = 1;
+        NBH_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count)
+    {
+        if (count > n)
+            alloc(count + count / 4);
+    }
+};
+
+enum Phase : int {
+    PH_KICK_DRIFT_BBOX = 0,
+    PH_ENCODE,
+    PH_SORT,
+    PH_GATHER,
+    PH_GROUPS,
+    PH_KARRAS,
+    PH_PLAN,
+    PH_OCTREE,
+    PH_COM,
+    PH_WALK,
+    PH_TRAVERSAL,
+    PH_KICK,
+    PH_COUNT
+};
+
+const char* kPhaseNames[PH_COUNT] = {
+    "kick_drift_bbox", "encode", "sort",  "gather",     "groups",    "karras",
+    "plan",            "octree", "com",   "walk_layout", "traversal", "kick"};
+
+// Host copy of the mid-step readback (pinned).
+struct Readback {
+    int ngroups;
+    int nnodes;
+    int errors[ERR_WORD_COUNT];
+    int pad;
+    double box[4];
+};
+
+} // namespace
+} // namespace nbh
+
+struct nbh_sim {
+    nbh_sim_config cfg{};
+    size_t n = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int state = 0; // 0 created, 1 ready, 2 failed
+    double time = 0.0;
+    size_t steps = 0;
+    bool timing = false;
+    int key_bytes = 8;
+    int passes = 8;
+
+    // particle state, original order
+    nbh::DevBuf<double> m, x, y, z, vx, vy, vz;
+    nbh::DevBuf<double> acc[2][3];
+    int cur = 0;
+
+    // build buffers sized for n
+    nbh::DevBuf<unsigned char> keys, keys_alt, unique;
+    nbh::DevBuf<uint32_t> idx, idx_alt;
+    nbh::DevBuf<double4> rec64;
+    nbh::DevBuf<int4> rec32;
+    nbh::DevBuf<int> gfirst, gcount;
+    nbh::DevBuf<int> left, right, tparent, rfirst, rlast, plen;
+    nbh::DevBuf<int> plevel, pcount, poffset;
+    // per-step cleared words: hist | sort status | group scan | plan scan |
+    // error words
+    nbh::DevBuf<uint32_t> zero;
+    size_t zero_words = 0;
+    uint32_t* hist = nullptr;
+    uint32_t* sort_status = nullptr;
+    unsigned long long* group_status = nullptr;
+    unsigned long long* plan_status = nullptr;
+    int* errors = nullptr;
+    // small device scalars
+    nbh::DevBuf<double> box;      // 4
+    nbh::DevBuf<double> partials; // bbox partials
+    nbh::DevBuf<int> scalars;     // [0] counter [1] ngroups [2] nnodes [3] wcount
+    nbh::DevBuf<unsigned long long> counters;
+
+    // octree (capacity grows)
+    size_t mcap = 0;
+    int nnodes = 0;
+    int ngroups = 0;
+    nbh::DevBuf<int> children, oparent, level, ofirst, ocount, pending,
+        child_count, size, local;
+    nbh::DevBuf<unsigned char> prefix;
+    nbh::DevBuf<double> mass, comx, comy, comz, cenx, ceny, cenz, half, crit;
+    nbh::DevBuf<double4> w64node, w64center;
+    nbh::DevBuf<double> w64crit;
+    nbh::DevBuf<int4> w64link, w32node, w32link, w32center;
+
+    nbh::Readback* rb = nullptr;
+    hipEvent_t ev[nbh::PH_COUNT + 1] = {};
+    double phase_ms[nbh::PH_COUNT] = {};
+
+    ~nbh_sim()
+    {
+        if (rb)
+            (void)hipHostFree(rb);
+        for (auto& e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (own_stream && stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    nbh_octree_view octree_view()
+    {
+        nbh_octree_view o{};
+        o.n_particles = static_cast<int>(n);
+        o.n_leaves = ngroups;
+        o.n_nodes = nnodes;
+        o.root = 0;
+        o.children = children.p;
+        o.parent = oparent.p;
+        o.level = level.p;
+        o.prefix = prefix.p;
+        o.first_particle = ofirst.p;
+        o.particle_count = ocount.p;
+        o.mass = mass.p;
+        o.com_x = comx.p;
+        o.com_y = comy.p;
+        o.com_z = comz.p;
+        o.center_x = cenx.p;
+        o.center_y = ceny.p;
+        o.center_z = cenz.p;
+        o.half_size = half.p;
+        o.pending_children = pending.p;
+        o.child_count = child_count.p;
+        return o;
+    }
+
+    void mark(int phase)
+    {
+        if (timing)
+            NBH_HIP(hipEventRecord(ev[phase], stream));
+    }
+};
+
+namespace nbh {
+namespace {
+
+bool mixed(const nbh_sim* s) { return s->cfg.precision == NBH_PRECISION_MIXED; }
+
+void alloc_state(nbh_sim* s)
+{
+    const size_t n = s->n;
+    for (auto* b : {&s->m, &s->x, &s->y, &s->z, &s->vx, &s->vy, &s->vz})
+        b->alloc(n);
+    for (auto& a : s->acc)
+        for (auto& c : a)
+            c.alloc(n);
+    s->keys.alloc(n * s->key_bytes);
+    s->keys_alt.alloc(n * s->key_bytes);
+    s->unique.alloc(n * s->key_bytes);
+    s->idx.alloc(n);
+    s->idx_alt.alloc(n);
+    s->rec64.alloc(n);
+    if (mixed(s))
+        s->rec32.alloc(n);
+    s->gfirst.alloc(n);
+    s->gcount.alloc(n);
+    s->left.alloc(n);
+    s->right.alloc(n);
+    s->rfirst.alloc(n);
+    s->rlast.alloc(n);
+    s->plen.alloc(n);
+    s->tparent.alloc(2 * n);
+    s->plevel.alloc(2 * n);
+    s->pcount.alloc(2 * n);
+    s->poffset.alloc(2 * n);
+    const size_t hist_w = 8 * 256;
+    const size_t sort_w = sort_status_words(n, s->passes);
+    const size_t grp_w = 2 * scan_status_words(n);
+    const size_t plan_w = 2 * scan_status_words(2 * n);
+    const size_t err_w = 64;
+    s->zero_words = hist_w + sort_w + grp_w + plan_w + err_w;
+    s->zero_words = (s->zero_words + 3) & ~size_t(3);
+    s->zero.alloc(s->zero_words + 8);
+    uint32_t* z = s->zero.p;
+    s->hist = z;
+    s->sort_status = z + hist_w;
+    // 64-bit words must be 8-byte aligned
+    size_t off = hist_w + sort_w;
+    off = (off + 1) & ~size_t(1);
+    s->group_status = reinterpret_cast<unsigned long long*>(z + off);
+    off += grp_w;
+    s->plan_status = reinterpret_cast<unsigned long long*>(z + off);
+    off += plan_w;
+    s->errors = reinterpret_cast<int*>(z + off);
+    s->zero_words = off + err_w;
+    s->box.alloc(4);
+    s->partials.alloc(bbox_partial_doubles());
+    s->scalars.alloc(16);
+    s->counters.alloc(2);
+    NBH_HIP(hipMemsetAsync(s->scalars.p, 0, 16 * sizeof(int), s->stream));
+    NBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->rb), sizeof(Readback),
+                          hipHostMallocDefault));
+    for (auto& e : s->ev)
+        NBH_HIP(hipEventCreate(&e));
+}
+
+void ensure_octree(nbh_sim* s, size_t m)
+{
+    if (m <= s->mcap)
+        return;
+    const size_t cap = m + m / 4 + 1024;
+    s->children.alloc(8 * cap);
+    for (auto* b : {&s->oparent, &s->level, &s->ofirst, &s->ocount, &s->pending,
+                    &s->child_count, &s->size, &s->local})
+        b->alloc(cap);
+    s->prefix.alloc(cap * s->key_bytes);
+    for (auto* b : {&s->mass, &s->comx, &s->comy, &s->comz, &s->cenx, &s->ceny,
+                    &s->cenz, &s->half, &s->crit})
+        b->alloc(cap);
+    if (mixed(s)) {
+        s->w32node.alloc(cap);
+        s->w32link.alloc(cap);
+        if (s->cfg.theta >= kContainsTheta)
+            s->w32center.alloc(cap);
+    } else {
+        s->w64node.alloc(cap);
+        s->w64crit.alloc(cap);
+        s->w64link.alloc(cap);
+        if (s->cfg.theta >= kContainsTheta)
+            s->w64center.alloc(cap);
+    }
+    s->mcap = cap;
+}
+
+template <int BITS>
+void rebuild(nbh_sim* s, bool kick_drift)
+{
+    using K = typename Layout<BITS>::Key;
+    hipStream_t st = s->stream;
+    const size_t n = s->n;
+    const int ni = static_cast<int>(n);
+    double* const* a = nullptr;
+    (void)a;
+    s->mark(PH_KICK_DRIFT_BBOX);
+    launch_fill_u32(st, s->zero.p, 0u, s->zero_words);
+    unsigned* counter = reinterpret_cast<unsigned*>(s->scalars.p);
+    if (kick_drift) {
+        auto& c = s->acc[s->cur];
+        launch_kick_drift_bbox(st, s->x.p, s->y.p, s->z.p, s->vx.p, s->vy.p,
+                               s->vz.p, c[0].p, c[1].p, c[2].p, n,
+                               s->cfg.time_step, s->box.p, s->partials.p,
+                               counter, s->errors + ERR_BBOX);
+    } else {
+        nbh_context tmp;
+        tmp.stream = st;
+        launch_bbox(&tmp, s->x.p, s->y.p, s->z.p, n, s->box.p, s->partials.p,
+                    counter, s->errors + ERR_BBOX);
+    }
+    s->mark(PH_ENCODE);
+    K* keys = reinterpret_cast<K*>(s->keys.p);
+    launch_encode<BITS>(st, s->x.p, s->y.p, s->z.p, n, s->box.p, keys,
+                        s->idx.p, s->hist);
+    s->mark(PH_SORT);
+    launch_radix_passes<K>(st, keys, s->idx.p, reinterpret_cast<K*>(s->keys_alt.p),
+                           s->idx_alt.p, n, s->passes, s->hist, s->sort_status);
+    s->mark(PH_GATHER);
+    launch_gather(st, s->idx.p, s->m.p, s->x.p, s->y.p, s->z.p, n, s->rec64.p,
+                  mixed(s) ? s->rec32.p : nullptr, s->box.p,
+                  s->cfg.gravitational_constant);
+    s->mark(PH_GROUPS);
+    int* d_ng = s->scalars.p + 1;
+    int* d_nm = s->scalars.p + 2;
+    K* unique = reinterpret_cast<K*>(s->unique.p);
+    launch_leaf_groups<K>(st, keys, ni, unique, s->gfirst.p, s->gcount.p, d_ng,
+                          s->group_status);
+    s->mark(PH_KARRAS);
+    launch_karras<K>(st, unique, d_ng, ni, s->left.p, s->right.p, s->tparent.p,
+                     s->rfirst.p, s->rlast.p, s->plen.p);
+    s->mark(PH_PLAN);
+    launch_plan_scan<BITS>(st, d_ng, ni, s->tparent.p, s->plen.p, unique,
+                           s->plevel.p, s->pcount.p, s->poffset.p, d_nm,
+                           s->errors + ERR_PLAN, s->plan_status);
+    // The one mid-step readback: group and node counts, stage errors, box.
+    NBH_HIP(hipMemcpyAsync(&s->rb->ngroups, d_ng, 2 * sizeof(int),
+                           hipMemcpyDeviceToHost, st));
+    NBH_HIP(hipMemcpyAsync(s->rb->errors, s->errors,
+                           ERR_WORD_COUNT * sizeof(int), hipMemcpyDeviceToHost,
+                           st));
+    NBH_HIP(hipMemcpyAsync(s->rb->box, s->box.p, 4 * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    NBH_HIP(hipStreamSynchronize(st));
+    if (s->rb->errors[ERR_BBOX] != 0)
+        fail(NBH_DOMAIN_ERROR,
+             "Particle positions must be finite to compute a bounding box.");
+    if (s->rb->errors[ERR_PLAN] != 0)
+        fail(NBH_RUNTIME_ERROR, stage_message(ERR_PLAN, s->rb->errors[ERR_PLAN]),
+             s->rb->errors[ERR_PLAN]);
+    s->ngroups = s->rb->ngroups;
+    if (s->ngroups <= 0 || s->ngroups > ni)
+        fail(NBH_RUNTIME_ERROR, "Leaf grouping produced an invalid group count");
+    if (s->rb->nnodes <= 0)
+        fail(NBH_OVERFLOW_ERROR,
+             "The computed octree node count does not fit in int");
+    s->nnodes = s->rb->nnodes;
+    ensure_octree(s, static_cast<size_t>(s->nnodes));
+    s->mark(PH_OCTREE);
+    nbh_octree_view o = s->octree_view();
+    launch_octree_build<BITS>(st, o, s->ngroups, s->tparent.p, s->rfirst.p,
+                              unique, s->gfirst.p, s->gcount.p, s->plevel.p,
+                              s->pcount.p, s->poffset.p, s->errors + ERR_OCTREE,
+                              true);
+    s->mark(PH_COM);
+    launch_climb<BITS>(st, o, s->rec64.p, s->cfg.theta, s->box.p, s->size.p,
+                       s->local.p, s->crit.p, s->errors + ERR_PHYSICS);
+    s->mark(PH_WALK);
+    const bool contains = s->cfg.theta >= kContainsTheta;
+    launch_walk_build(st, o, false, mixed(s), s->size.p, s->local.p, s->crit.p,
+                      s->cfg.theta, s->rec64.p, s->rec32.p, s->box.p,
+                      s->cfg.gravitational_constant, contains, s->w64node.p,
+                      s->w64crit.p, s->w64link.p, s->w64center.p, s->w32node.p,
+                      s->w32link.p, s->w32center.p, s->scalars.p + 3);
+}
+
+void forces(nbh_sim* s, int buffer, unsigned long long* counters)
+{
+    hipStream_t st = s->stream;
+    auto& a = s->acc[buffer];
+    const int ni = static_cast<int>(s->n);
+    const bool contains = s->cfg.theta >= kContainsTheta;
+    const double eps = s->cfg.softening;
+    if (!counters)
+        s->mark(PH_TRAVERSAL);
+    if (mixed(s)) {
+        const double scale = 2147483648.0 / s->rb->box[3];
+        const double eq = eps * scale;
+        launch_walk_mixed(st, s->w32node.p, s->w32link.p, s->w32center.p,
+                          s->scalars.p + 3, s->rec32.p, s->idx.p, ni,
+                          static_cast<float>(eq * eq), contains, eps > 0.0,
+                          a[0].p, a[1].p, a[2].p, s->errors + ERR_TRAVERSAL,
+                          counters);
+    } else {
+        launch_walk_fp64(st, s->w64node.p, s->w64crit.p, s->w64link.p,
+                         s->w64center.p, s->scalars.p + 3, s->rec64.p, s->idx.p,
+                         ni, eps * eps, s->cfg.gravitational_constant, contains,
+                         a[0].p, a[1].p, a[2].p, s->errors + ERR_TRAVERSAL,
+                         counters);
+    }
+}
+
+// Second synchronisation of a step: device errors of the octree, the
+// mass/COM climb and the traversal.
+void finish(nbh_sim* s)
+{
+    NBH_HIP(hipMemcpyAsync(s->rb->errors, s->errors,
+                           ERR_WORD_COUNT * sizeof(int), hipMemcpyDeviceToHost,
+                           s->stream));
+    NBH_HIP(hipStreamSynchronize(s->stream));
+    for (int w : {ERR_OCTREE, ERR_PHYSICS, ERR_TRAVERSAL}) {
+        const int code = s->rb->errors[w];
+        if (code != 0)
+            fail(NBH_RUNTIME_ERROR, stage_message(w, code), code);
+    }
+    if (s->timing) {
+        for (int p = 0; p < PH_COUNT; ++p) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, s->ev[p], s->ev[p + 1]) == hipSuccess)
+                s->phase_ms[p] = ms;
+            else
+                s->phase_ms[p] = 0.0;
+        }
+    }
+}
+
+void do_rebuild(nbh_sim* s, bool kick_drift)
+{
+    if (s->cfg.bits_per_axis == 10)
+        rebuild<10>(s, kick_drift);
+    else
+        rebuild<21>(s, kick_drift);
+}
+
+void validate_config(const nbh_sim_config* c)
+{
+    if (!c)
+        fail(NBH_INVALID_ARGUMENT, "config must not be null");
+    if (!std::isfinite(c->time_step) || c->time_step <= 0.0)
+        fail(NBH_INVALID_ARGUMENT, "Simulation timeStep must be finite and positive");
+    if (!std::isfinite(c->theta) || c->theta < 0.0)
+        fail(NBH_INVALID_ARGUMENT, "Simulation theta must be finite and non-negative");
+    if (!std::isfinite(c->softening) || c->softening < 0.0)
+        fail(NBH_INVALID_ARGUMENT,
+             "Simulation softening must be finite and non-negative");
+    if (!std::isfinite(c->gravitational_constant) || c->gravitational_constant <= 0.0)
+        fail(NBH_INVALID_ARGUMENT, "Gravitational constant must be finite and positive");
+    if (c->precision != NBH_PRECISION_FP64 && c->precision != NBH_PRECISION_MIXED)
+        fail(NBH_INVALID_ARGUMENT, "precision must be NBH_PRECISION_FP64 or _MIXED");
+    if (c->bits_per_axis != 10 && c->bits_per_axis != 21)
+        fail(NBH_INVALID_ARGUMENT, "bits_per_axis must be 10 or 21");
+}
+
+} // namespace
+} // namespace nbh
+
+using namespace nbh;
+
+extern "C" {
+
+int nbh_phase_count(void) { return PH_COUNT; }
+
+const char* nbh_phase_name(int phase)
+{
+    return (phase >= 0 && phase < PH_COUNT) ? kPhaseNames[phase] : "";
+}
+
+int nbh_sim_create(nbh_sim** out, const nbh_sim_config* config, size_t count,
+                   const double* h_mass, const double* h_x, const double* h_y,
+                   const double* h_z, const double* h_vx, const double* h_vy,
+                   const double* h_vz, void* hip_stream)
+{
+    return guard([&] {
+        if (!out)
+            fail(NBH_INVALID_ARGUMENT, "output pointer must not be null");
+        validate_config(config);
+        if (count == 0)
+            fail(NBH_INVALID_ARGUMENT, "Simulation requires at least one particle");
+        if (count > static_cast<size_t>(std::numeric_limits<int>::max()) / 2 ||
+            count >= (1u << 30))
+            fail(NBH_OVERFLOW_ERROR, "Particle count exceeds the tree representation");
+        if (!h_mass || !h_x || !h_y || !h_z || !h_vx || !h_vy || !h_vz)
+            fail(NBH_INVALID_ARGUMENT, "particle arrays must not be null");
+        auto* s = new nbh_sim();
+        try {
+            s->cfg = *config;
+            s->n = count;
+            s->key_bytes = config->bits_per_axis == 10 ? 4 : 8;
+            s->passes = config->bits_per_axis == 10 ? Layout<10>::sort_passes
+                                                    : Layout<21>::sort_passes;
+            if (hip_stream) {
+                s->stream = static_cast<hipStream_t>(hip_stream);
+            } else {
+                NBH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+                s->own_stream = true;
+            }
+            alloc_state(s);
+            const size_t bytes = count * sizeof(double);
+            const double* src[7] = {h_mass, h_x, h_y, h_z, h_vx, h_vy, h_vz};
+            nbh::DevBuf<double>* dst[7] = {&s->m, &s->x, &s->y, &s->z,
+                                           &s->vx, &s->vy, &s->vz};
+            for (int k = 0; k < 7; ++k)
+                NBH_HIP(hipMemcpyAsync(dst[k]->p, src[k], bytes,
+                                       hipMemcpyHostToDevice, s->stream));
+            for (auto& a : s->acc)
+                for (auto& c : a)
+                    launch_fill_f64(s->stream, c.p, 0.0, count);
+            NBH_HIP(hipStreamSynchronize(s->stream));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int nbh_sim_destroy(nbh_sim* sim)
+{
+    return guard([&] { delete sim; });
+}
+
+int nbh_sim_initialize(nbh_sim* s)
+{
+    return guard([&] {
+        if (!s)
+            fail(NBH_INVALID_ARGUMENT, "simulation must not be null");
+        if (s->state != 0)
+            fail(NBH_LOGIC_ERROR, "Simulation can only be initialized once");
+        try {
+            do_rebuild(s, false);
+            forces(s, 1 - s->cur, nullptr);
+            s->mark(PH_KICK);
+            s->mark(PH_COUNT);
+            finish(s);
+            s->cur = 1 - s->cur;
+            s->state = 1;
+        } catch (...) {
+            s->state = 2;
+            throw;
+        }
+    });
+}
+
+int nbh_sim_step(nbh_sim* s)
+{
+    return guard([&] {
+        if (!s)
+            fail(NBH_INVALID_ARGUMENT, "simulation must not be null");
+        if (s->state != 1)
+            fail(NBH_LOGIC_ERROR, "Simulation must be initialized before stepping");
+        try {
+            do_rebuild(s, true);
+            const int next = 1 - s->cur;
+            forces(s, next, nullptr);
+            s->mark(PH_KICK);
+            auto& a = s->acc[next];
+            launch_kick(s->stream, s->vx.p, s->vy.p, s->vz.p, a[0].p, a[1].p,
+                        a[2].p, s->n, s->cfg.time_step);
+            s->mark(PH_COUNT);
+            finish(s);
+            s->cur = next;
+            s->time += s->cfg.time_step;
+            ++s->steps;
+        } catch (...) {
+            s->state = 2;
+            throw;
+        }
+    });
+}
+
+int nbh_sim_run(nbh_sim* s, size_t steps)
+{
+    if (!s) {
+        set_last_error(0, "simulation must not be null");
+        return NBH_INVALID_ARGUMENT;
+    }
+    if (s->state != 1) {
+        set_last_error(0, "Simulation must be initialized before running");
+        return NBH_LOGIC_ERROR;
+    }
+    for (size_t k = 0; k < steps; ++k) {
+        const int r = nbh_sim_step(s);
+        if (r != NBH_OK)
+            return r;
+    }
+    return NBH_OK;
+}
+
+double nbh_sim_time(const nbh_sim* s) { return s ? s->time : 0.0; }
+size_t nbh_sim_step_number(const nbh_sim* s) { return s ? s->steps : 0; }
+
+int nbh_sim_particles(nbh_sim* s, nbh_particles_view* out)
+{
+    return guard([&] {
+        if (!s || !out)
+            fail(NBH_INVALID_ARGUMENT, "arguments must not be null");
+        out->count = s->n;
+        out->mass = s->m.p;
+        out->x = s->x.p;
+        out->y = s->y.p;
+        out->z = s->z.p;
+        out->vx = s->vx.p;
+        out->vy = s->vy.p;
+        out->vz = s->vz.p;
+    });
+}
+
+int nbh_sim_accelerations(nbh_sim* s, double** d_ax, double** d_ay,
+                          double** d_az)
+{
+    return guard([&] {
+        if (!s || !d_ax || !d_ay || !d_az)
+            fail(NBH_INVALID_ARGUMENT, "arguments must not be null");
+        *d_ax = s->acc[s->cur][0].p;
+        *d_ay = s->acc[s->cur][1].p;
+        *d_az = s->acc[s->cur][2].p;
+    });
+}
+
+int nbh_sim_structure_view(nbh_sim* s, nbh_sim_structure* out)
+{
+    return guard([&] {
+        if (!s || !out)
+            fail(NBH_INVALID_ARGUMENT, "arguments must not be null");
+        if (s->state == 0)
+            fail(NBH_LOGIC_ERROR, "Simulation has no structure before initialize");
+        std::memset(out, 0, sizeof(*out));
+        out->n_particles = static_cast<int>(s->n);
+        out->n_groups = s->ngroups;
+        out->n_octree_nodes = s->nnodes;
+        out->bits_per_axis = s->cfg.bits_per_axis;
+        out->sorted_keys = s->keys.p;
+        out->sorted_indices = s->idx.p;
+        out->unique_keys = s->unique.p;
+        out->group_first = s->gfirst.p;
+        out->group_count = s->gcount.p;
+        out->tree.n_leaves = s->ngroups;
+        out->tree.n_internal_nodes = s->ngroups - 1;
+        out->tree.left = s->left.p;
+        out->tree.right = s->right.p;
+        out->tree.parent = s->tparent.p;
+        out->tree.range_first = s->rfirst.p;
+        out->tree.range_last = s->rlast.p;
+        out->tree.prefix_length = s->plen.p;
+        out->octree = s->octree_view();
+        out->radix_level = s->plevel.p;
+        out->edge_node_count = s->pcount.p;
+        out->edge_node_offset = s->poffset.p;
+        out->box = s->box.p;
+    });
+}
+
+int nbh_sim_set_timing(nbh_sim* s, int enabled)
+{
+    return guard([&] {
+        if (!s)
+            fail(NBH_INVALID_ARGUMENT, "simulation must not be null");
+        s->timing = enabled != 0;
+    });
+}
+
+int nbh_sim_phase_times(nbh_sim* s, double* ms, int capacity)
+{
+    if (!s || !ms)
+        return NBH_INVALID_ARGUMENT;
+    const int k = capacity < PH_COUNT ? capacity : PH_COUNT;
+    for (int p = 0; p < k; ++p)
+        ms[p] = s->phase_ms[p];
+    return PH_COUNT;
+}
+
+int nbh_sim_count_interactions(nbh_sim* s, uint64_t* pp, uint64_t* pn)
+{
+    return guard([&] {
+        if (!s || !pp || !pn)
+            fail(NBH_INVALID_ARGUMENT, "arguments must not be null");
+        if (s->state != 1)
+            fail(NBH_LOGIC_ERROR, "Simulation must be initialized");
+        launch_fill_u64(s->stream, s->counters.p, 0ull, 2);
+        forces(s, s->cur, s->counters.p);
+        unsigned long long h[2];
+        NBH_HIP(hipMemcpyAsync(h, s->counters.p, sizeof(h),
+                               hipMemcpyDeviceToHost, s->stream));
+        NBH_HIP(hipStreamSynchronize(s->stream));
+        *pp = h[0];
+        *pn = h[1];
+    });
+}
+
+} // extern "C"

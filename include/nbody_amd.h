@@ -1,0 +1,348 @@
+/*
+ * nbody_amd.h -- C ABI of the MI355X Barnes-Hut step pipeline.
+ *
+ * Every entry point takes plain pointers and sizes.  Arrays named d_* live in
+ * device (HBM) memory; arrays named h_* live in host memory.  Every function
+ * returns an nbh_status; on failure nbh_last_error_message() describes the
+ * failure and nbh_last_device_code() holds the device-detected error code
+ * (the first-wins error word of the failing kernel), or 0.
+ *
+ * The C++ drop-in headers (amsc-nbody_amd/cpp/nbody/) rebuild the
+ * reference's classes and free functions on top of this ABI and rethrow each
+ * status as the exception class the reference throws.
+ *
+ * Morton key width is a runtime parameter `bits_per_axis`:
+ *   10 -> 30-bit keys stored in uint32_t (reference layout, 2 padding bits,
+ *         octree depth <= 10; src/radix_to_octree.hpp:10-19),
+ *   21 -> 63-bit keys stored in uint64_t (1 padding bit, depth <= 21).
+ * Functions that take `const void* keys` read uint32_t keys for 10 and
+ * uint64_t keys for 21.
+ *
+ * Streams: every call is ordered on the stream of its nbh_context (the null
+ * stream when the context was created with stream == NULL) and returns after
+ * the work is complete, as the reference's wrappers do
+ * (each ends in cudaDeviceSynchronize).  The simulation object runs a whole
+ * step with a single host synchronisation.
+ */
+#ifndef NBODY_AMD_H
+#define NBODY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum nbh_status {
+    NBH_OK = 0,
+    NBH_INVALID_ARGUMENT = -1, /* std::invalid_argument in the reference */
+    NBH_LOGIC_ERROR = -2,      /* std::logic_error                      */
+    NBH_RUNTIME_ERROR = -3,    /* std::runtime_error (device codes)     */
+    NBH_DOMAIN_ERROR = -4,     /* std::domain_error (non-finite input)  */
+    NBH_OVERFLOW_ERROR = -5,   /* std::overflow_error                   */
+    NBH_HIP_ERROR = -6,        /* HIP runtime failure (runtime_error)   */
+    NBH_OUT_OF_MEMORY = -7     /* allocation failure (runtime_error)    */
+} nbh_status;
+
+const char* nbh_last_error_message(void);
+int nbh_last_device_code(void);
+const char* nbh_version(void);
+
+/* --------------------------------------------------------------- context */
+/* A context binds a stream and a grow-only device workspace (sort buffers,
+ * scan carries, error words).  Not thread-safe: one context per host thread. */
+typedef struct nbh_context nbh_context;
+
+int nbh_context_create(nbh_context** out, void* hip_stream);
+int nbh_context_destroy(nbh_context* ctx);
+
+/* --------------------------------------------------------- bounding box */
+/* Replaces Bbox::compute_box (gpu/cuda_kernels/globalbounding.cu:94-144).
+ * Writes [centre_x, centre_y, centre_z, side] to d_box (device) and, when
+ * h_box != NULL, to h_box.  count == 0 -> NBH_INVALID_ARGUMENT; a non-finite
+ * coordinate -> NBH_DOMAIN_ERROR. */
+int nbh_bbox_compute(nbh_context* ctx, const double* d_x, const double* d_y,
+                     const double* d_z, size_t count, double* d_box,
+                     double* h_box);
+
+/* ------------------------------------------------------------ Morton keys */
+/* Replaces MortonKeys::build (gpu/cuda_kernels/morton.cu:160-225): normalise,
+ * quantise, interleave, then a stable sort of (key, index) pairs.
+ * d_box is the 4-double device box of nbh_bbox_compute.  Keys are written in
+ * ascending order to d_keys and the matching original indices to d_indices. */
+int nbh_morton_keys(nbh_context* ctx, const double* d_x, const double* d_y,
+                    const double* d_z, size_t count, const double* d_box,
+                    int bits_per_axis, void* d_keys, uint32_t* d_indices);
+
+/* Stable LSD radix sort of (key, value) pairs in place (the primitive behind
+ * thrust::stable_sort_by_key at morton.cu:221-224).  key_bits is the number
+ * of significant low key bits (keys must be < 2^key_bits); key_bytes 4 or 8. */
+int nbh_radix_sort_pairs(nbh_context* ctx, void* d_keys, uint32_t* d_values,
+                         size_t count, int key_bytes, int key_bits);
+
+/* ------------------------------------------------------ Morton leaf groups */
+/* Replaces buildMortonLeafGroups (gpu/cuda_kernels/morton_leaf_groups.cu:58-103):
+ * run-length encode sorted keys into unique keys and counts, exclusive scan of
+ * the counts into firstParticle.  key_bytes 4 or 8.  *h_n_groups receives the
+ * group count. */
+int nbh_leaf_groups_build(nbh_context* ctx, const void* d_sorted_keys,
+                          int key_bytes, int n_particles, int capacity,
+                          void* d_unique_keys, int* d_first_particle,
+                          int* d_particle_count, int* h_n_groups);
+
+/* ------------------------------------------------------- Karras radix tree */
+typedef struct nbh_tree_view {
+    int n_leaves;
+    int n_internal_nodes;
+    int* left;          /* n_leaves-1 */
+    int* right;         /* n_leaves-1 */
+    int* parent;        /* 2*n_leaves-1 */
+    int* range_first;   /* n_leaves-1 */
+    int* range_last;    /* n_leaves-1 */
+    int* prefix_length; /* n_leaves-1 */
+    double* mass;       /* 2*n_leaves-1 (legacy centre-of-mass API only) */
+    double* com_x;
+    double* com_y;
+    double* com_z;
+    int* visit_count;   /* n_leaves-1 (legacy centre-of-mass API only) */
+} nbh_tree_view;
+
+/* Replaces buildTree (gpu/cuda_kernels/tree_builder.cu:303-352): Karras 2012
+ * binary radix tree over sorted keys; leaves are 0..n-1, internal node i is
+ * n+i, the root is node n.  key_bytes 4 (delta = clz32, ties 32+clz(i^j)) or 8
+ * (delta = clz64, ties 64+clz(i^j)). */
+int nbh_tree_build(nbh_context* ctx, const nbh_tree_view* tree,
+                   const void* d_sorted_keys, int key_bytes, int n_leaves);
+
+/* Replace computeCenterOfMass / computeGroupedCenterOfMass
+ * (gpu/cuda_kernels/tree_builder.cu:374-503), the binary-tree mass and centre
+ * of mass kept for API completeness.  d_first/d_count NULL -> one particle per
+ * leaf. */
+int nbh_tree_center_of_mass(nbh_context* ctx, const nbh_tree_view* tree,
+                            const int* d_first_particle,
+                            const int* d_particle_count,
+                            const uint32_t* d_sorted_indices,
+                            const double* d_mass, const double* d_x,
+                            const double* d_y, const double* d_z);
+
+/* ----------------------------------------------- radix -> octree planning */
+/* Replaces buildRadixToOctreePlan (gpu/cuda_kernels/radix_to_octree.cu:174-244).
+ * n_radix_nodes = 2*n_leaves-1.  Writes radix level, edge node count and the
+ * exclusive scan of the counts; *h_n_octree_nodes = 1 + sum(counts). */
+int nbh_radix_to_octree_plan(nbh_context* ctx, const nbh_tree_view* tree,
+                             const void* d_unique_keys, int bits_per_axis,
+                             int* d_radix_level, int* d_edge_node_count,
+                             int* d_edge_node_offset, int* h_n_octree_nodes);
+
+/* ----------------------------------------------------------------- octree */
+typedef struct nbh_octree_view {
+    int n_particles;
+    int n_leaves;
+    int n_nodes;
+    int root;
+    int* children;        /* 8*n_nodes, node-major: children[node*8+octant] */
+    int* parent;
+    int* level;
+    void* prefix;         /* uint32_t (10 bits/axis) or uint64_t (21) */
+    int* first_particle;
+    int* particle_count;
+    double* mass;
+    double* com_x;
+    double* com_y;
+    double* com_z;
+    double* center_x;
+    double* center_y;
+    double* center_z;
+    double* half_size;
+    int* pending_children;
+    int* child_count;
+} nbh_octree_view;
+
+/* Replaces buildSparseOctreeTopology (gpu/cuda_kernels/octree_builder.cu:254-362). */
+int nbh_octree_build_topology(nbh_context* ctx, const nbh_octree_view* octree,
+                              const nbh_tree_view* tree,
+                              const void* d_unique_keys,
+                              const int* d_group_first,
+                              const int* d_group_count, int n_groups,
+                              int n_particles, int bits_per_axis,
+                              const int* d_radix_level,
+                              const int* d_edge_node_count,
+                              const int* d_edge_node_offset);
+
+/* Replaces computeOctreeGeometry (gpu/cuda_kernels/octree_geometry.cu:137-173).
+ * h_box = host [centre_x, centre_y, centre_z, side]. */
+int nbh_octree_geometry(nbh_context* ctx, const nbh_octree_view* octree,
+                        int bits_per_axis, const double* h_box);
+
+/* Replaces computeOctreeMassAndCenterOfMass
+ * (gpu/cuda_kernels/octree_physics.cu:283-337). Particle arrays in original
+ * order, d_sorted_indices maps sorted position -> original index. */
+int nbh_octree_mass_com(nbh_context* ctx, const nbh_octree_view* octree,
+                        const uint32_t* d_sorted_indices, const double* d_mass,
+                        const double* d_x, const double* d_y,
+                        const double* d_z);
+
+/* ------------------------------------------------------------ Barnes-Hut */
+enum { NBH_ORDER_ORIGINAL = 0, NBH_ORDER_MORTON = 1 };
+
+/* Replaces computeBarnesHutAcceleration (gpu/cuda_kernels/barnes_hut.cu:325-391):
+ * FP64 stackless traversal, MAC d > s/theta + delta, exact leaves, octants
+ * visited 7 -> 0.  particle_order NBH_ORDER_MORTON means m/x/y/z are already
+ * in sorted (Morton) order.  Accelerations are written in original order. */
+int nbh_bh_acceleration(nbh_context* ctx, const nbh_octree_view* octree,
+                        const uint32_t* d_sorted_indices, const double* d_mass,
+                        const double* d_x, const double* d_y,
+                        const double* d_z, double* d_ax, double* d_ay,
+                        double* d_az, size_t particle_count, double theta,
+                        double softening, double gravitational_constant,
+                        int particle_order);
+
+/* ------------------------------------------------------------- leapfrog */
+/* Replace leapfrogKickDrift / leapfrogKick (gpu/cuda_kernels/integrator.cu:109-145). */
+int nbh_leapfrog_kick_drift(nbh_context* ctx, double* d_x, double* d_y,
+                            double* d_z, double* d_vx, double* d_vy,
+                            double* d_vz, const double* d_ax,
+                            const double* d_ay, const double* d_az,
+                            size_t count, double dt);
+int nbh_leapfrog_kick(nbh_context* ctx, double* d_vx, double* d_vy,
+                      double* d_vz, const double* d_ax, const double* d_ay,
+                      const double* d_az, size_t count, double dt);
+
+/* Replaces gatherMortonOrderedParticleData (gpu/cuda_kernels/simulation.cu:33-107). */
+int nbh_gather_morton(nbh_context* ctx, const uint32_t* d_sorted_indices,
+                      const double* d_mass, const double* d_x,
+                      const double* d_y, const double* d_z, double* d_out_mass,
+                      double* d_out_x, double* d_out_y, double* d_out_z,
+                      size_t count);
+
+/* ------------------------------------------------------------ simulation */
+enum { NBH_PRECISION_FP64 = 0, NBH_PRECISION_MIXED = 1 };
+
+typedef struct nbh_sim_config {
+    double time_step;             /* SimulationConfig::timeStep  */
+    double theta;                 /* SimulationConfig::theta     */
+    double softening;             /* SimulationConfig::softening */
+    double gravitational_constant;/* default 6.67430e-11 (src/particle.hpp:16) */
+    int precision;                /* NBH_PRECISION_FP64 or _MIXED */
+    int bits_per_axis;            /* 10 or 21 */
+} nbh_sim_config;
+
+typedef struct nbh_sim nbh_sim;
+
+typedef struct nbh_particles_view {
+    size_t count;
+    double* mass;
+    double* x;
+    double* y;
+    double* z;
+    double* vx;
+    double* vy;
+    double* vz;
+} nbh_particles_view;
+
+/* Replaces Simulation::Simulation (src/simulation.cpp:21-43,70-74): validates the
+ * config and copies the particle state (host arrays h_* of length count) into
+ * device memory owned by the simulation. */
+int nbh_sim_create(nbh_sim** out, const nbh_sim_config* config, size_t count,
+                   const double* h_mass, const double* h_x, const double* h_y,
+                   const double* h_z, const double* h_vx, const double* h_vy,
+                   const double* h_vz, void* hip_stream);
+int nbh_sim_destroy(nbh_sim* sim);
+/* Simulation::initialize / step / run (src/simulation.cpp:78-127). */
+int nbh_sim_initialize(nbh_sim* sim);
+int nbh_sim_step(nbh_sim* sim);
+int nbh_sim_run(nbh_sim* sim, size_t steps);
+double nbh_sim_time(const nbh_sim* sim);
+size_t nbh_sim_step_number(const nbh_sim* sim);
+/* Simulation::particles (src/simulation.cpp:133-135): device pointers. */
+int nbh_sim_particles(nbh_sim* sim, nbh_particles_view* out);
+/* Current accelerations (original order) of the last force evaluation. */
+int nbh_sim_accelerations(nbh_sim* sim, double** d_ax, double** d_ay,
+                          double** d_az);
+/* Structures of the last rebuild, for parity checks: sorted keys/indices,
+ * leaf groups, radix tree and octree (device pointers owned by the sim). */
+typedef struct nbh_sim_structure {
+    int n_particles;
+    int n_groups;
+    int n_octree_nodes;
+    int bits_per_axis;
+    const void* sorted_keys;
+    const uint32_t* sorted_indices;
+    const void* unique_keys;
+    const int* group_first;
+    const int* group_count;
+    nbh_tree_view tree;
+    nbh_octree_view octree;
+    const int* radix_level;
+    const int* edge_node_count;
+    const int* edge_node_offset;
+    const double* box; /* device [cx, cy, cz, side] */
+} nbh_sim_structure;
+int nbh_sim_structure_view(nbh_sim* sim, nbh_sim_structure* out);
+
+/* Phase timings of the last step in milliseconds (HIP events on the sim's
+ * stream), in the order of nbh_phase_names(). Returns the phase count. */
+int nbh_phase_count(void);
+const char* nbh_phase_name(int phase);
+int nbh_sim_phase_times(nbh_sim* sim, double* ms, int capacity);
+int nbh_sim_set_timing(nbh_sim* sim, int enabled);
+/* Recompute the current forces once more and count the particle-particle and
+ * particle-node interactions of that evaluation (not part of a step). */
+int nbh_sim_count_interactions(nbh_sim* sim, uint64_t* pp, uint64_t* pn);
+
+/* -------------------------------------------------------------- adjoint */
+/* Differentiable K-step leapfrog with Barnes-Hut forces, tree topology fixed
+ * within each force evaluation.  Forward runs `steps` KDK steps from the
+ * initial state (host arrays), storing checkpoints every `checkpoint_every`
+ * steps, and returns the final positions/velocities.  Backward takes the
+ * adjoints of the final positions and velocities and returns the adjoints of
+ * the initial positions, velocities and masses, recomputing each segment from
+ * its checkpoint. */
+typedef struct nbh_adjoint nbh_adjoint;
+int nbh_adjoint_create(nbh_adjoint** out, const nbh_sim_config* config,
+                       size_t count, int steps, int checkpoint_every,
+                       void* hip_stream);
+int nbh_adjoint_destroy(nbh_adjoint* adj);
+int nbh_adjoint_forward(nbh_adjoint* adj, const double* h_mass,
+                        const double* h_pos /* 3*count, xyz planar */,
+                        const double* h_vel /* 3*count */,
+                        double* h_pos_out, double* h_vel_out);
+int nbh_adjoint_backward(nbh_adjoint* adj, const double* h_pos_bar,
+                         const double* h_vel_bar, double* h_pos0_bar,
+                         double* h_vel0_bar, double* h_mass_bar);
+/* Single-evaluation pair: a = BH(x, m) and its vector-Jacobian product. */
+int nbh_bh_force_forward(nbh_context* ctx, const nbh_sim_config* config,
+                         size_t count, const double* h_mass,
+                         const double* h_pos, double* h_acc);
+int nbh_bh_force_backward(nbh_context* ctx, const nbh_sim_config* config,
+                          size_t count, const double* h_mass,
+                          const double* h_pos, const double* h_acc_bar,
+                          double* h_pos_bar, double* h_mass_bar);
+
+/* ------------------------------------------------------------ workloads */
+/* Seeded synthetic particle sets (host arrays of length count).
+ * kind: 0 Plummer (a=1, M=1, truncated at 10a, equal masses)
+ *       1 uniform cube [-1,1]^3, lognormal(0,0.5) masses normalised to 1,
+ *         Gaussian velocities sigma 0.1
+ *       2 exponential disk (80%) + Hernquist bulge (20%)
+ *       3 256 Gaussian clumps in [-10,10]^3, cold start
+ *       4 Gaussian blob N(0,1)^3, velocities sigma 0.01, equal masses */
+int nbh_generate(int kind, size_t count, uint64_t seed, double* h_mass,
+                 double* h_x, double* h_y, double* h_z, double* h_vx,
+                 double* h_vy, double* h_vz);
+
+/* FP64 direct sum on `n_targets` targets (indices into the particle arrays),
+ * softened, on the device: the accuracy reference for large runs. */
+int nbh_direct_sum(nbh_context* ctx, const double* d_mass, const double* d_x,
+                   const double* d_y, const double* d_z, size_t count,
+                   const uint32_t* d_targets, size_t n_targets,
+                   double softening, double gravitational_constant,
+                   double* d_ax, double* d_ay, double* d_az);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_AMD_H */
