@@ -319,7 +319,9 @@ __global__ __launch_bounds__(256) void gather_kernel(
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const uint32_t p = idx[i];
+    uint32_t p = idx[i];
+    if (p >= n)
+        p = 0; // unreachable for a permutation; keeps every read in bounds
     const double px = x[p], py = y[p], pz = z[p], pm = m[p];
     rec64[i] = make_double4(px, py, pz, pm);
     if (rec32) {

@@ -99,6 +99,10 @@ __global__ __launch_bounds__(256) void materialize_kernel(
                 return;
             }
             parent_node = 1 + poffset[anc] + ac - 1;
+            if (parent_node <= 0 || parent_node >= o.n_nodes) {
+                raise_error(err, 6);
+                return;
+            }
             break;
         }
         anc = tparent[anc];

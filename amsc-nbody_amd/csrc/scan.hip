@@ -174,6 +174,8 @@ __global__ __launch_bounds__(kScanThreads) void leaf_groups_kernel(
         if (i >= n)
             break;
         const int g = v[j].a - 1;
+        if (g < 0 || g >= n)
+            break;
         const bool head = (i == 0 || k[j + 1] != k[j]);
         const bool tail = (i == n - 1 || k[j + 1] != k[j + 2]);
         if (head) {

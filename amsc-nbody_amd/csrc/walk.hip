@@ -103,6 +103,10 @@ __global__ __launch_bounds__(256) void climb_kernel(
     int v = node;
     while (v != 0) {
         const int p = o.parent[v];
+        if (p < 0 || p >= o.n_nodes) {
+            raise_error(err, 5);
+            return;
+        }
         const int required = o.child_count[p];
         const int done = fetch_add_acq_rel(&o.pending_children[p], 1) + 1;
         if (done < required)
@@ -223,7 +227,7 @@ __global__ __launch_bounds__(256) void generic_sizes_kernel(
         int off = 1;
         for (int oc = 7; oc >= 0; --oc) {
             const int c = o.children[p * 8 + oc];
-            if (c == -1)
+            if (c <= 0 || c >= o.n_nodes)
                 continue;
             local[c] = off;
             off += size[c];
@@ -270,10 +274,10 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
     if (v >= o.n_nodes)
         return;
     const int pre = preorder(o, local, v);
-    if (pre < 0)
+    if (pre < 0 || pre >= o.n_nodes)
         return;
     if (v == 0)
-        *wcount = size[0];
+        *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
     const int skip = pre + size[v];
     const int pc = o.particle_count[v];
     const double h = o.half_size[v];
@@ -363,6 +367,17 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
             w32center[pre] = make_int4(qx, qy, qz, qh);
         }
     }
+}
+
+// Every link starts as "error 4, skip to the end", so a position that the
+// build leaves unwritten (a node off the root's tree) stops the walk with a
+// topology error instead of reading an undefined record.
+__global__ __launch_bounds__(256) void link_sentinel_kernel(int4* __restrict__ link,
+                                                            int4 v, int n)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+        link[i] = v;
 }
 
 // ------------------------------------------------------------ traversal
@@ -519,6 +534,10 @@ __global__ __launch_bounds__(kWalkThreads) void walk_fp64_kernel(
     }
     if (valid) {
         const uint32_t o = idx[t];
+        if (o >= static_cast<uint32_t>(n)) {
+            raise_error(err, 3);
+            return;
+        }
         ax_out[o] = ax;
         ay_out[o] = ay;
         az_out[o] = az;
@@ -683,6 +702,10 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     }
     if (valid) {
         const uint32_t o = idx[t];
+        if (o >= static_cast<uint32_t>(n)) {
+            raise_error(err, 3);
+            return;
+        }
         ax_out[o] = ax.value();
         ay_out[o] = ay.value();
         az_out[o] = az.value();
@@ -699,6 +722,8 @@ __global__ __launch_bounds__(256) void direct_kernel(
 {
     __shared__ double s_red[3][4];
     const uint32_t t = targets[blockIdx.x];
+    if (t >= n)
+        return;
     const double tx = x[t], ty = y[t], tz = z[t];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     bool sing = false;
@@ -763,7 +788,15 @@ void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
                        int4* w32link, int4* w32center, int* wcount)
 {
     const unsigned blocks = blocks_for(o.n_nodes, 256);
-#define NBH_WALK_ARGS                                                            \
+    const int M = o.n_nodes;
+    if (mixed)
+        link_sentinel_kernel<<<blocks, 256, 0, s>>>(
+            w32link, make_int4(0, M, KIND_ERROR + 4, 0), M);
+    else
+        link_sentinel_kernel<<<blocks, 256, 0, s>>>(
+            w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M);
+    NBH_HIP(hipGetLastError());
+#define NBH_WALK_ARGS                                                          \
     o, size, local, crit, theta, rec64, rec32, box, G, need_center, w64node,     \
         w64crit, w64link, w64center, w32node, w32link, w32center, wcount
     if (generic && !mixed)

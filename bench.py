@@ -1,0 +1,197 @@
+"""Benchmark: Barnes-Hut leapfrog steps on one MI355X.
+
+Workload (default): Plummer sphere, N = 4,194,304, theta 0.5, softening 1e-3,
+dt 1e-4, G = 1, 63-bit Morton keys.  One step = kick-drift + bounding box,
+Morton keys, radix sort, leaf groups, Karras tree, radix->octree plan,
+octree, mass/COM, walk layout, traversal, closing kick -- the reference's
+Simulation::step (src/simulation.cpp:100-119) -- with inputs resident in HBM.
+
+Prints one JSON line: Mparticle-steps/s of the mixed-FP32 pipeline, the FP64
+pipeline's rate and the speed-up, the accuracy of the mixed forces against
+the FP64 pipeline and an FP64 direct sum, momentum/energy drift, the
+traversal's achieved VALU FLOP/s against the MI355X FP32 peak, and the CPU
+oracle timed on a bounded sample of the same workload.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "amsc-nbody_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+FP32_VALU_PEAK_TFLOPS = 157.3
+FP64_VALU_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+FLOPS_PER_INTERACTION = 20
+
+
+def rel_err(a, b):
+    a = np.stack(a)
+    b = np.stack(b)
+    r = np.sqrt(((a - b) ** 2).sum(0)) / np.sqrt((b ** 2).sum(0))
+    return float(np.sqrt((r ** 2).mean())), float(r.max())
+
+
+def momentum(s):
+    return np.array([math.fsum(s["m"] * s[k]) for k in ("vx", "vy", "vz")])
+
+
+def time_steps(nb, torch, sim, steps):
+    """Runs `steps` steps; returns seconds and mean phase times (ms)."""
+    phases = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+        pt = sim.phase_times()
+        if phases is None:
+            phases = {k: 0.0 for k in pt}
+        for k, v in pt.items():
+            phases[k] += v / steps
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, phases
+
+
+def cpu_baseline(p, args):
+    """The oracle (the reference algorithm in FP64 C++/OpenMP) on the same
+    particles: one full rebuild plus the forces of a bounded sample of
+    Morton-ordered targets, scaled to one step."""
+    import oracle
+    n = len(p["m"])
+    sample = min(args.cpu_targets, n)
+    sim = oracle.Sim(p, args.dt, args.theta, args.softening, 1.0, 21)
+    t0 = time.perf_counter()
+    sim.rebuild()
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sim.partial_forces(0, sample)
+    t_pf = time.perf_counter() - t0
+    t_forces = max(t_pf - t_build, 1e-9)
+    step = t_build + t_forces * n / sample
+    return {"value": n / step / 1e6, "unit": "Mparticle-steps/s",
+            "cores": oracle.threads(), "kind": "port",
+            "sample": f"one rebuild at N={n} ({t_build:.2f} s) + forces of {sample} "
+                      f"Morton-ordered targets ({t_forces:.2f} s), scaled to N"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=4_194_304)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--softening", type=float, default=1e-3)
+    ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--workload", default="plummer")
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--direct-targets", type=int, default=8192)
+    ap.add_argument("--cpu-targets", type=int, default=32768)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import nbody_amd as nb
+
+    n = args.n
+    p = nb.generate(args.workload, n, 42)
+    common = dict(gravitational_constant=1.0, bits_per_axis=21)
+
+    mixed = nb.Simulation(p, args.dt, args.theta, args.softening,
+                          precision=nb.PRECISION_MIXED, **common)
+    mixed.set_timing(True)
+    mixed.initialize()
+    a_mixed0 = mixed.accelerations()
+    pp, pn = mixed.count_interactions()
+    interactions = pp + pn
+
+    # accuracy at t = 0 against the FP64 pipeline (bit-exact with the oracle)
+    # and against an FP64 direct sum on sampled targets
+    acc = {}
+    fp64_rate = None
+    if not args.no_fp64:
+        f64 = nb.Simulation(p, args.dt, args.theta, args.softening,
+                            precision=nb.PRECISION_FP64, **common)
+        f64.set_timing(True)
+        f64.initialize()
+        a64 = f64.accelerations()
+        acc["rms_vs_fp64_bh"], acc["max_vs_fp64_bh"] = rel_err(a_mixed0, a64)
+    rng = np.random.default_rng(0)
+    targets = np.sort(rng.choice(n, min(args.direct_targets, n), replace=False)).astype(np.uint32)
+    dev = [torch.from_numpy(p[k]).cuda() for k in ("m", "x", "y", "z")]
+    exact = [t.cpu().numpy() for t in
+             nb.direct_sum(*dev, torch.from_numpy(targets.view(np.int32)).cuda(),
+                           args.softening, 1.0)]
+    acc["rms_vs_direct"], acc["max_vs_direct"] = rel_err([a[targets] for a in a_mixed0], exact)
+    if not args.no_fp64:
+        acc["fp64_rms_vs_direct"], acc["fp64_max_vs_direct"] = rel_err(
+            [a[targets] for a in a64], exact)
+    del dev
+
+    s0 = mixed.state()
+    p0 = momentum(s0)
+    for _ in range(args.warmup):
+        mixed.step()
+    secs, phases = time_steps(nb, torch, mixed, args.steps)
+    s1 = mixed.state()
+    p1 = momentum(s1)
+    ms_per_step = 1e3 * secs / args.steps
+    value = n * args.steps / secs / 1e6
+
+    if not args.no_fp64:
+        f64.step()  # warm-up
+        secs64, phases64 = time_steps(nb, torch, f64, args.fp64_steps)
+        fp64_rate = n * args.fp64_steps / secs64 / 1e6
+        f64.close()
+
+    trav_ms = phases["traversal"]
+    achieved = FLOPS_PER_INTERACTION * interactions / (trav_ms * 1e-3) / 1e12
+    line = {
+        "metric": "Mparticle-steps/s, N=4M theta=0.5: mixed-FP32 vs FP64 at <=1e-3 rms accel err",
+        "value": round(value, 3),
+        "unit": "Mparticle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 traversal / f64 build+integrate",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"{args.workload} N={n} theta={args.theta} "
+                               f"eps={args.softening} dt={args.dt} G=1 63-bit keys",
+                   "n_particles": n, "theta": args.theta, "softening": args.softening,
+                   "dt": args.dt, "precision": "mixed"},
+        "steps_per_s": round(1e3 / ms_per_step, 4),
+        "interactions_per_step": interactions,
+        "interactions_per_s": interactions * 1e3 / ms_per_step,
+        "fp64_value": None if fp64_rate is None else round(fp64_rate, 3),
+        "speedup_vs_fp64": None if fp64_rate is None else round(value / fp64_rate, 3),
+        "accuracy": {k: float(f"{v:.4g}") for k, v in acc.items()},
+        "momentum_drift": float(np.abs(p1 - p0).max()),
+        "phase_ms": {k: round(v, 4) for k, v in phases.items()},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "walk_mixed_kernel",
+                     "flops_per_interaction": FLOPS_PER_INTERACTION},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(p, args),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -283,7 +283,7 @@ typedef struct nbh_sim_structure {
 int nbh_sim_structure_view(nbh_sim* sim, nbh_sim_structure* out);
 
 /* Phase timings of the last step in milliseconds (HIP events on the sim's
- * stream), in the order of nbh_phase_names(). Returns the phase count. */
+ * stream), phase k named by nbh_phase_name(k). Returns the phase count. */
 int nbh_phase_count(void);
 const char* nbh_phase_name(int phase);
 int nbh_sim_phase_times(nbh_sim* sim, double* ms, int capacity);
@@ -291,35 +291,6 @@ int nbh_sim_set_timing(nbh_sim* sim, int enabled);
 /* Recompute the current forces once more and count the particle-particle and
  * particle-node interactions of that evaluation (not part of a step). */
 int nbh_sim_count_interactions(nbh_sim* sim, uint64_t* pp, uint64_t* pn);
-
-/* -------------------------------------------------------------- adjoint */
-/* Differentiable K-step leapfrog with Barnes-Hut forces, tree topology fixed
- * within each force evaluation.  Forward runs `steps` KDK steps from the
- * initial state (host arrays), storing checkpoints every `checkpoint_every`
- * steps, and returns the final positions/velocities.  Backward takes the
- * adjoints of the final positions and velocities and returns the adjoints of
- * the initial positions, velocities and masses, recomputing each segment from
- * its checkpoint. */
-typedef struct nbh_adjoint nbh_adjoint;
-int nbh_adjoint_create(nbh_adjoint** out, const nbh_sim_config* config,
-                       size_t count, int steps, int checkpoint_every,
-                       void* hip_stream);
-int nbh_adjoint_destroy(nbh_adjoint* adj);
-int nbh_adjoint_forward(nbh_adjoint* adj, const double* h_mass,
-                        const double* h_pos /* 3*count, xyz planar */,
-                        const double* h_vel /* 3*count */,
-                        double* h_pos_out, double* h_vel_out);
-int nbh_adjoint_backward(nbh_adjoint* adj, const double* h_pos_bar,
-                         const double* h_vel_bar, double* h_pos0_bar,
-                         double* h_vel0_bar, double* h_mass_bar);
-/* Single-evaluation pair: a = BH(x, m) and its vector-Jacobian product. */
-int nbh_bh_force_forward(nbh_context* ctx, const nbh_sim_config* config,
-                         size_t count, const double* h_mass,
-                         const double* h_pos, double* h_acc);
-int nbh_bh_force_backward(nbh_context* ctx, const nbh_sim_config* config,
-                          size_t count, const double* h_mass,
-                          const double* h_pos, const double* h_acc_bar,
-                          double* h_pos_bar, double* h_mass_bar);
 
 /* ------------------------------------------------------------ workloads */
 /* Seeded synthetic particle sets (host arrays of length count).

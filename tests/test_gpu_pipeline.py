@@ -1,0 +1,198 @@
+"""Simulation pipeline parity with the oracle's Simulation loop (GPU).
+
+FP64: Morton keys, permutation, octree topology, mass/COM, accelerations and
+the leapfrog state after several steps match the oracle bit for bit.
+Mixed: topology identical; accelerations within the stated tolerance of the
+FP64 oracle.  Large sizes: size-independent properties and accuracy against
+an FP64 direct sum.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+# Mixed precision vs the FP64 oracle at the same theta: rms and max relative
+# acceleration error bounds.
+MIXED_RMS = 1e-4
+MIXED_MAX = 5e-3
+
+
+def rel_err(a, b):
+    a = np.stack(a)
+    b = np.stack(b)
+    num = np.sqrt(((a - b) ** 2).sum(0))
+    den = np.sqrt((b ** 2).sum(0))
+    r = num / den
+    return float(np.sqrt((r ** 2).mean())), float(r.max())
+
+
+def make(nb, kind, n, seed=42):
+    return nb.generate(kind, n, seed)
+
+
+def compare_structure(g, o):
+    assert g["n_groups"] == o["n_groups"]
+    assert g["n_nodes"] == o["n_nodes"]
+    assert np.array_equal(g["keys"], o["keys"])
+    assert np.array_equal(g["indices"], o["indices"])
+    for f in ("children", "parent", "first", "count", "mass", "com_x", "com_y", "com_z"):
+        assert np.array_equal(g[f], o[f]), f
+
+
+@pytest.mark.parametrize("bits", [10, 21])
+@pytest.mark.parametrize("kind,n", [("plummer", 3000), ("uniform", 2500), ("clumps", 4096),
+                                    ("plummer", 1), ("plummer", 2)])
+def test_fp64_pipeline_bit_exact(nb, orc, bits, kind, n):
+    p = make(nb, kind, n)
+    cfg = dict(time_step=1e-3, theta=0.5, softening=0.01)
+    sim = nb.Simulation(p, gravitational_constant=1.0, precision=nb.PRECISION_FP64,
+                        bits_per_axis=bits, **cfg)
+    ref = orc.Sim(p, cfg["time_step"], cfg["theta"], cfg["softening"], 1.0, bits)
+    sim.initialize()
+    ref.initialize()
+    compare_structure(sim.structure_host(), ref.structure())
+    rs = ref.state()
+    a = sim.accelerations()
+    for k, f in enumerate(("ax", "ay", "az")):
+        assert np.array_equal(a[k], rs[f]), f
+    sim.run(3)
+    ref.step(3)
+    rs = ref.state()
+    st = sim.state()
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(st[f], rs[f]), f
+    a = sim.accelerations()
+    for k, f in enumerate(("ax", "ay", "az")):
+        assert np.array_equal(a[k], rs[f]), f
+    compare_structure(sim.structure_host(), ref.structure())
+    assert sim.step_number() == 3
+    assert abs(sim.time() - 3e-3) < 1e-15
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.8, 1.2])
+@pytest.mark.parametrize("kind", ["plummer", "clumps"])
+def test_mixed_pipeline_accuracy(nb, orc, theta, kind):
+    n = 20_000
+    p = make(nb, kind, n)
+    sim = nb.Simulation(p, 1e-4, theta, 1e-3, gravitational_constant=1.0,
+                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    ref = orc.Sim(p, 1e-4, theta, 1e-3, 1.0, 21)
+    sim.initialize()
+    ref.initialize()
+    g, o = sim.structure_host(), ref.structure()
+    assert np.array_equal(g["keys"], o["keys"]) and np.array_equal(g["children"], o["children"])
+    rs = ref.state()
+    rms, mx = rel_err(sim.accelerations(), [rs["ax"], rs["ay"], rs["az"]])
+    print(f"mixed vs fp64 oracle {kind} theta={theta}: rms={rms:.3e} max={mx:.3e}")
+    assert rms < MIXED_RMS and mx < MIXED_MAX
+
+
+def test_mixed_pipeline_without_softening(nb, orc):
+    p = make(nb, "uniform", 3000)
+    sim = nb.Simulation(p, 1e-4, 0.5, 0.0, gravitational_constant=1.0,
+                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    ref = orc.Sim(p, 1e-4, 0.5, 0.0, 1.0, 21)
+    sim.initialize()
+    ref.initialize()
+    rs = ref.state()
+    rms, mx = rel_err(sim.accelerations(), [rs["ax"], rs["ay"], rs["az"]])
+    assert rms < MIXED_RMS and mx < MIXED_MAX
+
+
+def test_mixed_steps_track_fp64(nb, orc):
+    n = 5000
+    p = make(nb, "plummer", n)
+    sim = nb.Simulation(p, 1e-3, 0.5, 0.01, gravitational_constant=1.0,
+                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    ref = orc.Sim(p, 1e-3, 0.5, 0.01, 1.0, 21)
+    sim.initialize()
+    ref.initialize()
+    sim.run(5)
+    ref.step(5)
+    st, rs = sim.state(), ref.state()
+    dx = np.sqrt((st["x"] - rs["x"]) ** 2 + (st["y"] - rs["y"]) ** 2 + (st["z"] - rs["z"]) ** 2)
+    assert dx.max() < 1e-8
+
+
+def test_simulation_lifecycle_errors(nb):
+    p = make(nb, "plummer", 100)
+    sim = nb.Simulation(p, 1e-3, 0.5, 0.01, gravitational_constant=1.0)
+    with pytest.raises(nb.LogicError):
+        sim.step()
+    with pytest.raises(nb.LogicError):
+        sim.run(1)
+    sim.initialize()
+    with pytest.raises(nb.LogicError):
+        sim.initialize()
+    sim.step()
+    assert sim.step_number() == 1
+
+
+@pytest.mark.parametrize("bad", [dict(time_step=0.0), dict(time_step=np.nan),
+                                 dict(theta=-0.1), dict(softening=-1.0),
+                                 dict(gravitational_constant=0.0)])
+def test_simulation_rejects_config(nb, bad):
+    p = make(nb, "plummer", 10)
+    cfg = dict(time_step=1e-3, theta=0.5, softening=0.01, gravitational_constant=1.0)
+    cfg.update(bad)
+    with pytest.raises(nb.InvalidArgument):
+        nb.Simulation(p, **cfg)
+
+
+def test_simulation_non_finite_marks_failed(nb):
+    p = make(nb, "plummer", 100)
+    p["vx"][7] = np.inf
+    sim = nb.Simulation(p, 1e-3, 0.5, 0.01, gravitational_constant=1.0)
+    sim.initialize()
+    with pytest.raises(nb.DomainError):
+        sim.step()
+    with pytest.raises(nb.LogicError):
+        sim.step()
+
+
+def test_isolated_particle_stays_put(nb):
+    p = {k: np.array([v]) for k, v in dict(m=1.0, x=1.0, y=2.0, z=3.0, vx=0.5, vy=0.0,
+                                            vz=0.0).items()}
+    sim = nb.Simulation(p, 0.1, 0.5, 0.01, gravitational_constant=1.0)
+    sim.initialize()
+    sim.run(10)
+    st = sim.state()
+    assert abs(st["x"][0] - 1.5) < 1e-14 and st["y"][0] == 2.0
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_large_properties_and_direct_sum(nb, precision):
+    n = 1 << 20
+    p = make(nb, "uniform", n, seed=7)
+    sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0, precision=precision,
+                        bits_per_axis=21)
+    sim.initialize()
+    s = sim.structure_host()
+    keys, idx = s["keys"], s["indices"]
+    assert np.all(keys[1:] >= keys[:-1])
+    assert np.array_equal(np.sort(idx), np.arange(n, dtype=np.uint32))
+    assert s["group_count"].sum() == n
+    occupied = s["count"] > 0
+    assert occupied.sum() == s["n_groups"]
+    assert abs(s["mass"][0] - 1.0) < 1e-12
+    ch = s["children"].reshape(-1, 8)
+    kids = ch[ch >= 0]
+    assert len(kids) == s["n_nodes"] - 1 and len(np.unique(kids)) == len(kids)
+    assert np.all(s["parent"][kids] == np.nonzero(ch >= 0)[0])
+    rng = np.random.default_rng(3)
+    targets = rng.choice(n, 1024, replace=False).astype(np.uint32)
+    v = sim.particles_view()
+    dev = [torch.from_numpy(p[k]).cuda() for k in ("m", "x", "y", "z")]
+    exact = [t.cpu().numpy() for t in
+             nb.direct_sum(*dev, torch.from_numpy(targets.view(np.int32)).cuda(), 1e-3, 1.0)]
+    a = sim.accelerations()
+    rms, mx = rel_err([a[k][targets] for k in range(3)], exact)
+    print(f"N=2^20 precision={precision}: BH vs direct rms={rms:.3e} max={mx:.3e}")
+    assert rms < 1e-2
+    pp, pn = sim.count_interactions()
+    assert pp > 0 and pn > 0
+    del v
