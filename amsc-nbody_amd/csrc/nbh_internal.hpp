@@ -229,11 +229,7 @@ void launch_leaf_init(hipStream_t s, const nbh_octree_view& o,
                       const double* y, const double* z, int* err);
 void launch_propagate(hipStream_t s, const nbh_octree_view& o, int* err);
 
-// walk.hip
-template <int BITS>
-void launch_climb(hipStream_t s, const nbh_octree_view& o, const double4* rec,
-                  double theta, const double* box, int* size, int* local,
-                  double* crit, int* err);
+// walk_layout.hip, walk_fp64.hip, walk_mixed.hip
 void launch_generic_sizes(hipStream_t s, const nbh_octree_view& o, int* size,
                           int* local, int* err);
 void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,

@@ -25,9 +25,14 @@ const char* traversal_message(int code);
 const char* stage_message(int word, int code);
 
 template <int BITS>
-void launch_climb(hipStream_t s, const nbh_octree_view& o, const double4* rec,
-                  double theta, const double* box, int* size, int* local,
-                  double* crit, int* err);
+void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
+                         bool mixed, const double4* rec64, const int4* rec32,
+                         double theta, const double* box, double G,
+                         bool need_center, bool soft, int* size, int* local,
+                         double* crit, int* pre, double4* w64node, double* w64crit,
+                         int4* w64link, double4* w64center, int4* w32node,
+                         int4* w32link, int4* w32center, int* wcount,
+                         hipEvent_t between);
 
 namespace {
 
@@ -138,7 +143,7 @@ struct nbh_sim {
     int nnodes = 0;
     int ngroups = 0;
     nbh::DevBuf<int> children, oparent, level, ofirst, ocount, pending,
-        child_count, size, local;
+        child_count, size, local, pre;
     nbh::DevBuf<unsigned char> prefix;
     nbh::DevBuf<double> mass, comx, comy, comz, cenx, ceny, cenz, half, crit;
     nbh::DevBuf<double4> w64node, w64center;
@@ -248,7 +253,7 @@ void alloc_state(nbh_sim* s)
     s->box.alloc(4);
     s->partials.alloc(bbox_partial_doubles());
     s->scalars.alloc(16);
-    s->counters.alloc(2);
+    s->counters.alloc(4);
     NBH_HIP(hipMemsetAsync(s->scalars.p, 0, 16 * sizeof(int), s->stream));
     NBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->rb), sizeof(Readback),
                           hipHostMallocDefault));
@@ -263,7 +268,7 @@ void ensure_octree(nbh_sim* s, size_t m)
     const size_t cap = m + m / 4 + 1024;
     s->children.alloc(8 * cap);
     for (auto* b : {&s->oparent, &s->level, &s->ofirst, &s->ocount, &s->pending,
-                    &s->child_count, &s->size, &s->local})
+                    &s->child_count, &s->size, &s->local, &s->pre})
         b->alloc(cap);
     s->prefix.alloc(cap * s->key_bytes);
     for (auto* b : {&s->mass, &s->comx, &s->comy, &s->comz, &s->cenx, &s->ceny,
@@ -360,17 +365,15 @@ void rebuild(nbh_sim* s, bool kick_drift)
     launch_octree_build<BITS>(st, o, s->ngroups, s->tparent.p, s->rfirst.p,
                               unique, s->gfirst.p, s->gcount.p, s->plevel.p,
                               s->pcount.p, s->poffset.p, s->errors + ERR_OCTREE,
-                              true);
+                              false);
     s->mark(PH_COM);
-    launch_climb<BITS>(st, o, s->rec64.p, s->cfg.theta, s->box.p, s->size.p,
-                       s->local.p, s->crit.p, s->errors + ERR_PHYSICS);
-    s->mark(PH_WALK);
     const bool contains = s->cfg.theta >= kContainsTheta;
-    launch_walk_build(st, o, false, mixed(s), s->size.p, s->local.p, s->crit.p,
-                      s->cfg.theta, s->rec64.p, s->rec32.p, s->box.p,
-                      s->cfg.gravitational_constant, contains, s->w64node.p,
-                      s->w64crit.p, s->w64link.p, s->w64center.p, s->w32node.p,
-                      s->w32link.p, s->w32center.p, s->scalars.p + 3);
+    launch_level_passes<BITS>(
+        st, o, Layout<BITS>::bits, mixed(s), s->rec64.p, s->rec32.p,
+        s->cfg.theta, s->box.p, s->cfg.gravitational_constant, contains,
+        s->cfg.softening > 0.0, s->size.p, s->local.p, s->crit.p, s->pre.p, s->w64node.p, s->w64crit.p,
+        s->w64link.p, s->w64center.p, s->w32node.p, s->w32link.p,
+        s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr);
 }
 
 void forces(nbh_sim* s, int buffer, unsigned long long* counters)
@@ -668,21 +671,24 @@ int nbh_sim_phase_times(nbh_sim* s, double* ms, int capacity)
     return PH_COUNT;
 }
 
-int nbh_sim_count_interactions(nbh_sim* s, uint64_t* pp, uint64_t* pn)
+int nbh_sim_count_interactions(nbh_sim* s, uint64_t* pp, uint64_t* pn,
+                               uint64_t* wave_visits)
 {
     return guard([&] {
         if (!s || !pp || !pn)
             fail(NBH_INVALID_ARGUMENT, "arguments must not be null");
         if (s->state != 1)
             fail(NBH_LOGIC_ERROR, "Simulation must be initialized");
-        launch_fill_u64(s->stream, s->counters.p, 0ull, 2);
+        launch_fill_u64(s->stream, s->counters.p, 0ull, 4);
         forces(s, s->cur, s->counters.p);
-        unsigned long long h[2];
+        unsigned long long h[4];
         NBH_HIP(hipMemcpyAsync(h, s->counters.p, sizeof(h),
                                hipMemcpyDeviceToHost, s->stream));
         NBH_HIP(hipStreamSynchronize(s->stream));
         *pp = h[0];
         *pn = h[1];
+        if (wave_visits)
+            *wave_visits = h[2];
     });
 }
 

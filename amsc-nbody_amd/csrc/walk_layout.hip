@@ -1,0 +1,506 @@
+// walk_layout.hip -- the derived depth-first layout the traversals walk.
+//
+// Layout.  The reference walks the octree per target with parent climbs and
+// 8-slot sibling scans, visiting octants 7 -> 0 (barnes_hut.cu:90-162).  Here
+// the same visiting order is laid out once per build: nodes are stored at
+// their depth-first pre-order position (children 7 -> 0), each record
+// carrying `skip` = pre + subtree size, the position that follows its
+// subtree.  "Open" is then pre + 1 and "done with this subtree" is skip: no
+// stack, no parent reads.
+//
+// Packets.  One wavefront owns 64 Morton-consecutive targets.  The wave
+// walks the layout in increasing position.  A lane is active at position i
+// when its own next position is <= i; active lanes evaluate the node exactly
+// as the per-target walk would (leaf -> exact particles, zero mass -> skip,
+// else MAC: accept -> monopole, reject -> open).  If any lane opens, the
+// wave steps to i + 1, else to skip[i].  Every lane therefore sees precisely
+// the node sequence of its own per-target walk, in the same order, while node
+// records are fetched once per wave with scalar loads.
+//
+// Opening test.  The reference accepts a node when the target is outside the
+// cell, d^2 > 0, theta > 0 and d > s/theta + delta (barnes_hut.cu:273-294).
+// For theta < 2/sqrt(3) the containment test never decides: inside the cell
+// d <= delta + sqrt(3) h < 2h/theta + delta.  The kernels therefore load
+// cell geometry only when theta >= 1.15.
+//
+// Precision.  FP64 reproduces the reference arithmetic (and is bit-exact
+// with the oracle).  Mixed evaluates the opening test and the interaction in
+// FP32 on exact integer differences of 31-bit fixed-point coordinates over
+// the bounding box -- coordinates relative to the target, so the rounding of
+// each separation is relative to the separation itself -- and sums each
+// target's acceleration with Kahan compensation.
+#include "octree_device.hpp"
+#include "walk_common.hpp"
+
+namespace nbh {
+
+namespace {
+
+// ------------------------------------------------- generic tree sizes
+// For octrees handed in through the C ABI: child ids must be valid (else
+// code 1, barnes_hut.cu:101-105) and every child must name its parent (else
+// code 4, barnes_hut.cu:150-154).  Writes the child counts to the scratch
+// array o.child_count.
+__global__ __launch_bounds__(256) void check_parents_kernel(nbh_octree_view o,
+                                                            int* err)
+{
+    const int node = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (node >= o.n_nodes)
+        return;
+    int count = 0;
+    for (int oc = 0; oc < 8; ++oc) {
+        const int c = o.children[node * 8 + oc];
+        if (c == -1)
+            continue;
+        if (c <= 0 || c >= o.n_nodes) {
+            raise_error(err, 1);
+            return;
+        }
+        if (o.parent[c] != node)
+            raise_error(err, 4);
+        ++count;
+    }
+    o.child_count[node] = count;
+}
+
+// Subtree sizes and pre-order offsets, started from every childless node.
+__global__ __launch_bounds__(256) void generic_sizes_kernel(
+    nbh_octree_view o, int* __restrict__ size, int* __restrict__ local,
+    int* err)
+{
+    const int node = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (node >= o.n_nodes)
+        return;
+    if (o.child_count[node] != 0)
+        return;
+    size[node] = 1;
+    int v = node;
+    int guard = 0;
+    while (v != 0) {
+        const int p = o.parent[v];
+        if (p < 0 || p >= o.n_nodes || ++guard > o.n_nodes) {
+            // not connected to the root: never visited by the traversal
+            return;
+        }
+        const int required = o.child_count[p];
+        const int done = fetch_add_acq_rel(&o.pending_children[p], 1) + 1;
+        if (done < required)
+            return;
+        if (done > required) {
+            raise_error(err, 4);
+            return;
+        }
+        int off = 1;
+        for (int oc = 7; oc >= 0; --oc) {
+            const int c = o.children[p * 8 + oc];
+            if (c <= 0 || c >= o.n_nodes)
+                continue;
+            local[c] = off;
+            off += size[c];
+        }
+        size[p] = off;
+        v = p;
+    }
+    local[0] = 0;
+}
+
+// Pre-order position of a node: the sum of the sibling offsets along its
+// path to the root.  Returns -1 when the path does not reach node 0.
+__device__ __forceinline__ int preorder(const nbh_octree_view& o,
+                                        const int* __restrict__ local, int v)
+{
+    int pre = 0;
+    int guard = 0;
+    while (v != 0) {
+        pre += local[v];
+        v = o.parent[v];
+        if (v < 0 || ++guard > 64)
+            return -1;
+    }
+    return pre;
+}
+
+// --------------------------------------------------------- walk records
+// Outputs of the walk build (records indexed by pre-order position).
+struct WalkOut {
+    double4* w64node;
+    double* w64crit;
+    int4* w64link;
+    double4* w64center;
+    int4* w32node;
+    int4* w32link;
+    int4* w32center;
+};
+
+// The record of node v at pre-order position `pre`.  GENERIC: a
+// caller-supplied octree, with the reference's per-node failure conditions
+// encoded as error kinds and the opening radius computed here.  Otherwise:
+// the pipeline's own octree with its precomputed opening radius.
+template <bool GENERIC, bool MIXED>
+__device__ __forceinline__ void write_record(
+    const nbh_octree_view& o, int v, int pre, const int* __restrict__ size,
+    const double* __restrict__ crit_in, double theta,
+    const double4* __restrict__ rec64, const int4* __restrict__ rec32,
+    const double* __restrict__ box, double G, bool need_center, bool soft,
+    const WalkOut& w)
+{
+    double4* w64node = w.w64node;
+    double* w64crit = w.w64crit;
+    int4* w64link = w.w64link;
+    double4* w64center = w.w64center;
+    int4* w32node = w.w32node;
+    int4* w32link = w.w32link;
+    int4* w32center = w.w32center;
+    const int skip = pre + size[v];
+    const int pc = o.particle_count[v];
+    const double h = o.half_size[v];
+    int kind;
+    int first = 0, count = 0;
+    if (GENERIC) {
+        const int nch = o.child_count[v];
+        if (!(h > 0.0)) {
+            kind = KIND_ERROR + 8;
+        } else if (pc > 0) {
+            first = o.first_particle[v];
+            count = pc;
+            if (nch != 0)
+                kind = KIND_ERROR + 7;
+            else if (first < 0 || pc > o.n_particles ||
+                     first > o.n_particles - pc)
+                kind = KIND_ERROR + 2;
+            else
+                kind = KIND_LEAF;
+        } else if (o.mass[v] == 0.0) {
+            kind = KIND_SKIP;
+        } else if (nch == 0) {
+            kind = KIND_CHILDLESS;
+        } else {
+            kind = KIND_INTERNAL;
+        }
+    } else {
+        if (pc > 0) {
+            kind = KIND_LEAF;
+            first = o.first_particle[v];
+            count = pc;
+        } else if (o.mass[v] == 0.0) {
+            kind = KIND_SKIP;
+        } else {
+            kind = KIND_INTERNAL;
+        }
+    }
+    const double m = o.mass[v];
+    const double cx = o.com_x[v], cy = o.com_y[v], cz = o.com_z[v];
+    double crit = 0.0;
+    if (kind == KIND_INTERNAL || kind == KIND_CHILDLESS) {
+        if (GENERIC) {
+            const double ox = cx - o.center_x[v];
+            const double oy = cy - o.center_y[v];
+            const double oz = cz - o.center_z[v];
+            const double o2 = fma(oz, oz, fma(ox, ox, oy * oy));
+            crit = (2.0 * h) / theta + sqrt(o2);
+        } else {
+            crit = crit_in[v];
+        }
+    }
+    if (!MIXED) {
+        if (kind == KIND_LEAF && count == 1)
+            w64node[pre] = rec64[first];
+        else
+            w64node[pre] = make_double4(cx, cy, cz, m);
+        w64crit[pre] = crit;
+        w64link[pre] = make_int4(skip, kind, first, count);
+        if (need_center)
+            w64center[pre] =
+                make_double4(o.center_x[v], o.center_y[v], o.center_z[v], h);
+    } else {
+        const double side = box[3];
+        const double scale = 2147483648.0 / side;
+        const double ox = box[0] - side / 2.0;
+        const double oy = box[1] - side / 2.0;
+        const double oz = box[2] - side / 2.0;
+        const double lim = 2147483647.0;
+        if (kind == KIND_LEAF && count == 1) {
+            w32node[pre] = rec32[first];
+        } else {
+            const int qx = static_cast<int>(fmin(fmax(rint((cx - ox) * scale), 0.0), lim));
+            const int qy = static_cast<int>(fmin(fmax(rint((cy - oy) * scale), 0.0), lim));
+            const int qz = static_cast<int>(fmin(fmax(rint((cz - oz) * scale), 0.0), lim));
+            const float gm = static_cast<float>(G * m * scale * scale);
+            w32node[pre] = make_int4(qx, qy, qz, __float_as_int(gm));
+        }
+        const double cq = crit * scale;
+        float crit2 = static_cast<float>(cq * cq);
+        // With positive softening a single-particle leaf and a zero-mass
+        // node are "always accept" monopoles (crit^2 = -1): the target's own
+        // particle and a zero mass both add exactly zero.
+        const bool fast = soft && ((kind == KIND_LEAF && count == 1) || kind == KIND_SKIP);
+        const int kind_out = fast ? KIND_INTERNAL : kind;
+        if (fast)
+            crit2 = -1.0f;
+        w32link[pre] = make_int4(__float_as_int(crit2), skip,
+                                 kind_out | (count << 8), first);
+        if (need_center) {
+            const int qx = static_cast<int>(rint((o.center_x[v] - ox) * scale));
+            const int qy = static_cast<int>(rint((o.center_y[v] - oy) * scale));
+            const int qz = static_cast<int>(rint((o.center_z[v] - oz) * scale));
+            const int qh = fast ? -1 : static_cast<int>(fmin(rint(h * scale), lim));
+            w32center[pre] = make_int4(qx, qy, qz, qh);
+        }
+    }
+}
+
+template <bool GENERIC, bool MIXED>
+__global__ __launch_bounds__(256) void walk_build_kernel(
+    nbh_octree_view o, const int* __restrict__ size,
+    const int* __restrict__ local, const double* __restrict__ crit_in,
+    double theta, const double4* __restrict__ rec64,
+    const int4* __restrict__ rec32, const double* __restrict__ box,
+    double G, bool need_center, WalkOut w, int* __restrict__ wcount)
+{
+    const int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= o.n_nodes)
+        return;
+    const int pre = preorder(o, local, v);
+    if (pre < 0 || pre >= o.n_nodes)
+        return;
+    if (v == 0)
+        *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
+    write_record<GENERIC, MIXED>(o, v, pre, size, crit_in, theta, rec64, rec32,
+                                 box, G, need_center, false, w);
+}
+
+// ------------------------------------------ level-synchronous passes
+// Bottom-up over one octree level: every child of a level-L node sits at
+// level L+1 and was completed by the previous launch, so no arrival
+// counters (and no cache write-back/invalidate per node) are needed.  A
+// node's values are the reference's: leaf mass/COM summed over its
+// particles in sorted order (octree_physics.cu:94-153), internal nodes
+// combined in octant order 0..7 (octree_physics.cu:211-243), cell geometry
+// from the prefix path (octree_geometry.cu:56-102), the opening radius
+// side/theta + |com - centre| (barnes_hut.cu:277-291); plus the subtree size
+// and the pre-order offsets of the children (octants 7 -> 0).
+template <int BITS>
+__global__ __launch_bounds__(256) void level_up_kernel(
+    nbh_octree_view o, int L, const double4* __restrict__ rec, double theta,
+    const double* __restrict__ box, int* __restrict__ size,
+    int* __restrict__ local, double* __restrict__ crit)
+{
+    using Key = typename Layout<BITS>::Key;
+    const double rcx = box[0], rcy = box[1], rcz = box[2], side = box[3];
+    const Key* prefix = static_cast<const Key*>(o.prefix);
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+         v < o.n_nodes; v += stride) {
+        if (o.level[v] != L)
+            continue;
+        double gx, gy, gz, gh;
+        int code = 0;
+        node_geometry<BITS>(prefix[v], L, rcx, rcy, rcz, side, gx, gy, gz, gh,
+                            code);
+        o.center_x[v] = gx;
+        o.center_y[v] = gy;
+        o.center_z[v] = gz;
+        o.half_size[v] = gh;
+        double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+        const int pc = o.particle_count[v];
+        int nch = 0;
+        if (pc > 0) {
+            const int f = o.first_particle[v];
+            for (int k = 0; k < pc; ++k) {
+                const double4 p = rec[f + k];
+                tm += p.w;
+                wx = fma(p.w, p.x, wx);
+                wy = fma(p.w, p.y, wy);
+                wz = fma(p.w, p.z, wz);
+            }
+            size[v] = 1;
+        } else {
+            const int4 c0 = reinterpret_cast<const int4*>(o.children)[2 * v];
+            const int4 c1 = reinterpret_cast<const int4*>(o.children)[2 * v + 1];
+            const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+            for (int oc = 0; oc < 8; ++oc) {
+                if (ch[oc] == -1)
+                    continue;
+                const double cm = o.mass[ch[oc]];
+                tm += cm;
+                wx = fma(cm, o.com_x[ch[oc]], wx);
+                wy = fma(cm, o.com_y[ch[oc]], wy);
+                wz = fma(cm, o.com_z[ch[oc]], wz);
+                ++nch;
+            }
+            int off = 1;
+#pragma unroll
+            for (int oc = 7; oc >= 0; --oc) {
+                if (ch[oc] == -1)
+                    continue;
+                local[ch[oc]] = off;
+                off += size[ch[oc]];
+            }
+            size[v] = off;
+        }
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        if (tm > 0.0) {
+            const double inv = 1.0 / tm;
+            cx = wx * inv;
+            cy = wy * inv;
+            cz = wz * inv;
+        }
+        o.mass[v] = tm;
+        o.com_x[v] = cx;
+        o.com_y[v] = cy;
+        o.com_z[v] = cz;
+        o.child_count[v] = nch;
+        o.pending_children[v] = nch;
+        if (pc <= 0) {
+            const double ox = cx - gx;
+            const double oy = cy - gy;
+            const double oz = cz - gz;
+            const double o2 = fma(oz, oz, fma(ox, ox, oy * oy));
+            crit[v] = (2.0 * gh) / theta + sqrt(o2);
+        }
+        if (v == 0)
+            local[0] = 0;
+    }
+}
+
+// Top-down over one level: pre-order position = parent's + own offset, then
+// the node's walk record.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void level_down_kernel(
+    nbh_octree_view o, int L, const int* __restrict__ size,
+    const int* __restrict__ local, int* __restrict__ pre_of,
+    const double* __restrict__ crit, double theta,
+    const double4* __restrict__ rec64, const int4* __restrict__ rec32,
+    const double* __restrict__ box, double G, bool need_center, bool soft, WalkOut w,
+    int* __restrict__ wcount)
+{
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+         v < o.n_nodes; v += stride) {
+        if (o.level[v] != L)
+            continue;
+        int pre = 0;
+        if (L == 0)
+            *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
+        else
+            pre = pre_of[o.parent[v]] + local[v];
+        pre_of[v] = pre;
+        if (pre < 0 || pre >= o.n_nodes)
+            continue;
+        write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32,
+                                   box, G, need_center, false, w);
+    }
+}
+
+// Every link starts as "error 4, skip to the end", so a position that the
+// build leaves unwritten (a node off the root's tree) stops the walk with a
+// topology error instead of reading an undefined record.
+__global__ __launch_bounds__(256) void link_sentinel_kernel(int4* __restrict__ link,
+                                                            int4 v, int n)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+        link[i] = v;
+}
+
+} // namespace
+
+void launch_generic_sizes(hipStream_t s, const nbh_octree_view& o, int* size,
+                          int* local, int* err)
+{
+    check_parents_kernel<<<blocks_for(o.n_nodes, 256), 256, 0, s>>>(o, err);
+    NBH_HIP(hipGetLastError());
+    generic_sizes_kernel<<<blocks_for(o.n_nodes, 256), 256, 0, s>>>(o, size,
+                                                                    local, err);
+    NBH_HIP(hipGetLastError());
+}
+
+void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
+                       bool mixed, const int* size, const int* local,
+                       const double* crit, double theta, const double4* rec64,
+                       const int4* rec32, const double* box, double G,
+                       bool need_center, double4* w64node, double* w64crit,
+                       int4* w64link, double4* w64center, int4* w32node,
+                       int4* w32link, int4* w32center, int* wcount)
+{
+    const unsigned blocks = blocks_for(o.n_nodes, 256);
+    const int M = o.n_nodes;
+    if (mixed)
+        link_sentinel_kernel<<<blocks, 256, 0, s>>>(
+            w32link, make_int4(0, M, KIND_ERROR + 4, 0), M);
+    else
+        link_sentinel_kernel<<<blocks, 256, 0, s>>>(
+            w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M);
+    NBH_HIP(hipGetLastError());
+    const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32link, w32center};
+#define NBH_WALK_ARGS o, size, local, crit, theta, rec64, rec32, box, G, need_center, w, wcount
+    if (generic && !mixed)
+        walk_build_kernel<true, false><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
+    else if (generic && mixed)
+        walk_build_kernel<true, true><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
+    else if (!mixed)
+        walk_build_kernel<false, false><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
+    else
+        walk_build_kernel<false, true><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
+#undef NBH_WALK_ARGS
+    NBH_HIP(hipGetLastError());
+}
+
+// The pipeline's mass/COM + walk layout: levels max_level..0 bottom-up, then
+// 0..max_level top-down.  Each launch strides over all nodes and handles
+// those of its level.
+template <int BITS>
+void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
+                         bool mixed, const double4* rec64, const int4* rec32,
+                         double theta, const double* box, double G,
+                         bool need_center, bool soft, int* size, int* local, double* crit,
+                         int* pre, double4* w64node, double* w64crit,
+                         int4* w64link, double4* w64center, int4* w32node,
+                         int4* w32link, int4* w32center, int* wcount,
+                         hipEvent_t between)
+{
+    unsigned blocks = blocks_for(o.n_nodes, 256);
+    if (blocks > 2048)
+        blocks = 2048;
+    for (int L = max_level; L >= 0; --L) {
+        level_up_kernel<BITS><<<blocks, 256, 0, s>>>(o, L, rec64, theta, box, size,
+                                                     local, crit);
+        NBH_HIP(hipGetLastError());
+    }
+    if (between)
+        NBH_HIP(hipEventRecord(between, s));
+    const int M = o.n_nodes;
+    if (mixed)
+        link_sentinel_kernel<<<blocks_for(M, 256), 256, 0, s>>>(
+            w32link, make_int4(0, M, KIND_ERROR + 4, 0), M);
+    else
+        link_sentinel_kernel<<<blocks_for(M, 256), 256, 0, s>>>(
+            w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M);
+    NBH_HIP(hipGetLastError());
+    const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32link, w32center};
+    for (int L = 0; L <= max_level; ++L) {
+        if (mixed)
+            level_down_kernel<true><<<blocks, 256, 0, s>>>(
+                o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
+                need_center, soft, w, wcount);
+        else
+            level_down_kernel<false><<<blocks, 256, 0, s>>>(
+                o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
+                need_center, soft, w, wcount);
+        NBH_HIP(hipGetLastError());
+    }
+}
+
+#define NBH_LEVEL_INST(B)                                                            \
+    template void launch_level_passes<B>(hipStream_t, const nbh_octree_view&, int, bool, \
+                                         const double4*, const int4*, double,           \
+                                         const double*, double, bool, bool, int*, int*, \
+                                         double*, int*, double4*, double*, int4*,       \
+                                         double4*, int4*, int4*, int4*, int*, hipEvent_t);
+NBH_LEVEL_INST(10)
+NBH_LEVEL_INST(21)
+#undef NBH_LEVEL_INST
+
+
+} // namespace nbh

@@ -169,7 +169,7 @@ _sig("nbh_phase_count", _i)
 _sig("nbh_phase_name", C.c_char_p, _i)
 _sig("nbh_sim_phase_times", _i, _p, C.POINTER(_d), _i)
 _sig("nbh_sim_set_timing", _i, _p, _i)
-_sig("nbh_sim_count_interactions", _i, _p, C.POINTER(_u64), C.POINTER(_u64))
+_sig("nbh_sim_count_interactions", _i, _p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64))
 _sig("nbh_generate", _i, _i, _sz, _u64, _p, _p, _p, _p, _p, _p, _p)
 _sig("nbh_direct_sum", _i, _p, _p, _p, _p, _p, _sz, _p, _sz, _d, _d, _p, _p, _p)
 
@@ -775,9 +775,12 @@ class Simulation:
                 out[k] = _copy_d2h(getattr(t, f), g - 1, np.int32)
         return out
 
-    def count_interactions(self):
-        pp, pn = _u64(0), _u64(0)
-        check(lib.nbh_sim_count_interactions(self.handle, C.byref(pp), C.byref(pn)))
+    def count_interactions(self, with_visits=False):
+        pp, pn, wv = _u64(0), _u64(0), _u64(0)
+        check(lib.nbh_sim_count_interactions(self.handle, C.byref(pp), C.byref(pn),
+                                             C.byref(wv)))
+        if with_visits:
+            return pp.value, pn.value, wv.value
         return pp.value, pn.value
 
 

@@ -114,7 +114,7 @@ def main():
     mixed.set_timing(True)
     mixed.initialize()
     a_mixed0 = mixed.accelerations()
-    pp, pn = mixed.count_interactions()
+    pp, pn, visits = mixed.count_interactions(with_visits=True)
     interactions = pp + pn
 
     # accuracy at t = 0 against the FP64 pipeline (bit-exact with the oracle)
@@ -178,6 +178,7 @@ def main():
         "steps_per_s": round(1e3 / ms_per_step, 4),
         "interactions_per_step": interactions,
         "interactions_per_s": interactions * 1e3 / ms_per_step,
+        "packet_efficiency": round(interactions / (64.0 * visits), 4) if visits else None,
         "fp64_value": None if fp64_rate is None else round(fp64_rate, 3),
         "speedup_vs_fp64": None if fp64_rate is None else round(value / fp64_rate, 3),
         "accuracy": {k: float(f"{v:.4g}") for k, v in acc.items()},

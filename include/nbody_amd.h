@@ -289,8 +289,11 @@ const char* nbh_phase_name(int phase);
 int nbh_sim_phase_times(nbh_sim* sim, double* ms, int capacity);
 int nbh_sim_set_timing(nbh_sim* sim, int enabled);
 /* Recompute the current forces once more and count the particle-particle and
- * particle-node interactions of that evaluation (not part of a step). */
-int nbh_sim_count_interactions(nbh_sim* sim, uint64_t* pp, uint64_t* pn);
+ * particle-node interactions of that evaluation (not part of a step), and,
+ * when wave_visits != NULL, the node visits of all wavefronts (mixed
+ * precision only; interactions / (64 * visits) is the packet efficiency). */
+int nbh_sim_count_interactions(nbh_sim* sim, uint64_t* pp, uint64_t* pn,
+                               uint64_t* wave_visits);
 
 /* ------------------------------------------------------------ workloads */
 /* Seeded synthetic particle sets (host arrays of length count).
