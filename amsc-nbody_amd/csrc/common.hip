@@ -142,6 +142,18 @@ int nbh_context_create(nbh_context** out, void* hip_stream)
     });
 }
 
+int nbh_copy_to_host(void* h_dst, const void* d_src, size_t bytes)
+{
+    return nbh::guard([&] {
+        if (bytes == 0)
+            return;
+        if (!h_dst || !d_src)
+            nbh::fail(NBH_INVALID_ARGUMENT, "copy pointers must not be null");
+        NBH_HIP(hipDeviceSynchronize());
+        NBH_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
 int nbh_context_destroy(nbh_context* ctx)
 {
     return nbh::guard([&] {

@@ -246,7 +246,8 @@ void launch_walk_fp64(hipStream_t s, const double4* wnode, const double* wcrit,
                       const uint32_t* idx, int n, double eps2, double G,
                       bool contains, double* ax, double* ay, double* az,
                       int* err, unsigned long long* counters);
-void launch_walk_mixed(hipStream_t s, const int4* wnode, const int4* wlink,
+// wpairs: interleaved {node, link} int4 pairs, 32 bytes per position.
+void launch_walk_mixed(hipStream_t s, const int4* wpairs,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx, int n,
                        float eps2, bool contains, bool soft, double* ax,

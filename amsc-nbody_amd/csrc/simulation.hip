@@ -148,7 +148,7 @@ struct nbh_sim {
     nbh::DevBuf<double> mass, comx, comy, comz, cenx, ceny, cenz, half, crit;
     nbh::DevBuf<double4> w64node, w64center;
     nbh::DevBuf<double> w64crit;
-    nbh::DevBuf<int4> w64link, w32node, w32link, w32center;
+    nbh::DevBuf<int4> w64link, w32node, w32center;
 
     nbh::Readback* rb = nullptr;
     hipEvent_t ev[nbh::PH_COUNT + 1] = {};
@@ -275,8 +275,7 @@ void ensure_octree(nbh_sim* s, size_t m)
                     &s->cenz, &s->half, &s->crit})
         b->alloc(cap);
     if (mixed(s)) {
-        s->w32node.alloc(cap);
-        s->w32link.alloc(cap);
+        s->w32node.alloc(2 * cap); // interleaved {node, link} pairs
         if (s->cfg.theta >= kContainsTheta)
             s->w32center.alloc(cap);
     } else {
@@ -354,7 +353,11 @@ void rebuild(nbh_sim* s, bool kick_drift)
              s->rb->errors[ERR_PLAN]);
     s->ngroups = s->rb->ngroups;
     if (s->ngroups <= 0 || s->ngroups > ni)
-        fail(NBH_RUNTIME_ERROR, "Leaf grouping produced an invalid group count");
+        fail(NBH_RUNTIME_ERROR,
+             "Leaf grouping produced an invalid group count (" +
+                 std::to_string(s->ngroups) + " groups, " +
+                 std::to_string(s->rb->nnodes) + " nodes for " +
+                 std::to_string(ni) + " particles)");
     if (s->rb->nnodes <= 0)
         fail(NBH_OVERFLOW_ERROR,
              "The computed octree node count does not fit in int");
@@ -372,7 +375,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
         st, o, Layout<BITS>::bits, mixed(s), s->rec64.p, s->rec32.p,
         s->cfg.theta, s->box.p, s->cfg.gravitational_constant, contains,
         s->cfg.softening > 0.0, s->size.p, s->local.p, s->crit.p, s->pre.p, s->w64node.p, s->w64crit.p,
-        s->w64link.p, s->w64center.p, s->w32node.p, s->w32link.p,
+        s->w64link.p, s->w64center.p, s->w32node.p, nullptr,
         s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr);
 }
 
@@ -388,7 +391,7 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters)
     if (mixed(s)) {
         const double scale = 2147483648.0 / s->rb->box[3];
         const double eq = eps * scale;
-        launch_walk_mixed(st, s->w32node.p, s->w32link.p, s->w32center.p,
+        launch_walk_mixed(st, s->w32node.p, s->w32center.p,
                           s->scalars.p + 3, s->rec32.p, s->idx.p, ni,
                           static_cast<float>(eq * eq), contains, eps > 0.0,
                           a[0].p, a[1].p, a[2].p, s->errors + ERR_TRAVERSAL,

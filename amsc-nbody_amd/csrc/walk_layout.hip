@@ -220,24 +220,28 @@ __device__ __forceinline__ void write_record(
         const double oz = box[2] - side / 2.0;
         const double lim = 2147483647.0;
         if (kind == KIND_LEAF && count == 1) {
-            w32node[pre] = rec32[first];
+            w32node[2 * pre] = rec32[first];
         } else {
             const int qx = static_cast<int>(fmin(fmax(rint((cx - ox) * scale), 0.0), lim));
             const int qy = static_cast<int>(fmin(fmax(rint((cy - oy) * scale), 0.0), lim));
             const int qz = static_cast<int>(fmin(fmax(rint((cz - oz) * scale), 0.0), lim));
             const float gm = static_cast<float>(G * m * scale * scale);
-            w32node[pre] = make_int4(qx, qy, qz, __float_as_int(gm));
+            w32node[2 * pre] = make_int4(qx, qy, qz, __float_as_int(gm));
         }
         const double cq = crit * scale;
         float crit2 = static_cast<float>(cq * cq);
         // With positive softening a single-particle leaf and a zero-mass
         // node are "always accept" monopoles (crit^2 = -1): the target's own
-        // particle and a zero mass both add exactly zero.
+        // particle and a zero mass both add exactly zero.  Records of every
+        // other non-internal kind carry crit^2 = +inf, so that the
+        // traversal's MAC never accepts them.
         const bool fast = soft && ((kind == KIND_LEAF && count == 1) || kind == KIND_SKIP);
         const int kind_out = fast ? KIND_INTERNAL : kind;
         if (fast)
             crit2 = -1.0f;
-        w32link[pre] = make_int4(__float_as_int(crit2), skip,
+        else if (kind != KIND_INTERNAL)
+            crit2 = __int_as_float(0x7F800000);
+        w32link[2 * pre] = make_int4(__float_as_int(crit2), skip,
                                  kind_out | (count << 8), first);
         if (need_center) {
             const int qx = static_cast<int>(rint((o.center_x[v] - ox) * scale));
@@ -283,7 +287,7 @@ template <int BITS>
 __global__ __launch_bounds__(256) void level_up_kernel(
     nbh_octree_view o, int L, const double4* __restrict__ rec, double theta,
     const double* __restrict__ box, int* __restrict__ size,
-    int* __restrict__ local, double* __restrict__ crit)
+    int* __restrict__ local, double* __restrict__ crit, bool collapse)
 {
     using Key = typename Layout<BITS>::Key;
     const double rcx = box[0], rcy = box[1], rcz = box[2], side = box[3];
@@ -303,7 +307,7 @@ __global__ __launch_bounds__(256) void level_up_kernel(
         o.half_size[v] = gh;
         double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
         const int pc = o.particle_count[v];
-        int nch = 0;
+        int nch = 0, only = 0;
         if (pc > 0) {
             const int f = o.first_particle[v];
             for (int k = 0; k < pc; ++k) {
@@ -327,9 +331,15 @@ __global__ __launch_bounds__(256) void level_up_kernel(
                 wx = fma(cm, o.com_x[ch[oc]], wx);
                 wy = fma(cm, o.com_y[ch[oc]], wy);
                 wz = fma(cm, o.com_z[ch[oc]], wz);
+                only = ch[oc];
                 ++nch;
             }
-            int off = 1;
+            // A collapsed single-child node takes no walk position of its
+            // own: its child starts at the node's position and overwrites
+            // the node's record on the next top-down level.  Only internal
+            // children and single-particle leaves qualify (a leaf holding
+            // several particles is summed exactly, never as a monopole).
+            int off = (collapse && nch == 1 && o.particle_count[only] <= 1) ? 0 : 1;
 #pragma unroll
             for (int oc = 7; oc >= 0; --oc) {
                 if (ch[oc] == -1)
@@ -372,8 +382,8 @@ __global__ __launch_bounds__(256) void level_down_kernel(
     const int* __restrict__ local, int* __restrict__ pre_of,
     const double* __restrict__ crit, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, double G, bool need_center, bool soft, WalkOut w,
-    int* __restrict__ wcount)
+    const double* __restrict__ box, double G, bool need_center, bool soft,
+    WalkOut w, int* __restrict__ wcount)
 {
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
     for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -389,7 +399,7 @@ __global__ __launch_bounds__(256) void level_down_kernel(
         if (pre < 0 || pre >= o.n_nodes)
             continue;
         write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32,
-                                   box, G, need_center, false, w);
+                                   box, G, need_center, soft, w);
     }
 }
 
@@ -397,11 +407,12 @@ __global__ __launch_bounds__(256) void level_down_kernel(
 // build leaves unwritten (a node off the root's tree) stops the walk with a
 // topology error instead of reading an undefined record.
 __global__ __launch_bounds__(256) void link_sentinel_kernel(int4* __restrict__ link,
-                                                            int4 v, int n)
+                                                            int4 v, int n,
+                                                            int stride)
 {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n)
-        link[i] = v;
+        link[static_cast<size_t>(i) * stride] = v;
 }
 
 } // namespace
@@ -426,14 +437,18 @@ void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
 {
     const unsigned blocks = blocks_for(o.n_nodes, 256);
     const int M = o.n_nodes;
+    // Mixed records are interleaved {node, link} pairs (see below).
+    (void)w32link;
+    int4* w32pair_link = w32node ? w32node + 1 : nullptr;
     if (mixed)
         link_sentinel_kernel<<<blocks, 256, 0, s>>>(
-            w32link, make_int4(0, M, KIND_ERROR + 4, 0), M);
+            w32pair_link, make_int4(0x7F800000, M, KIND_ERROR + 4, 0), M, 2);
     else
         link_sentinel_kernel<<<blocks, 256, 0, s>>>(
-            w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M);
+            w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M, 1);
     NBH_HIP(hipGetLastError());
-    const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32link, w32center};
+    const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32pair_link,
+                    w32center};
 #define NBH_WALK_ARGS o, size, local, crit, theta, rec64, rec32, box, G, need_center, w, wcount
     if (generic && !mixed)
         walk_build_kernel<true, false><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
@@ -460,12 +475,21 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          int4* w32link, int4* w32center, int* wcount,
                          hipEvent_t between)
 {
+    // Mixed records are interleaved {node, link} pairs: w32node is the array
+    // base and w32link = w32node + 1, both indexed at 2 * position.
+    (void)w32link;
+    int4* w32pair_link = w32node ? w32node + 1 : nullptr;
+    // Single-child nodes are collapsed into their child in the mixed walk:
+    // they carry the child's mass and centre of mass, and for theta below
+    // 2/sqrt(3) accepting the parent implies accepting the child, so every
+    // lane's interaction list is unchanged.
+    const bool collapse = mixed && theta < kContainsTheta;
     unsigned blocks = blocks_for(o.n_nodes, 256);
     if (blocks > 2048)
         blocks = 2048;
     for (int L = max_level; L >= 0; --L) {
         level_up_kernel<BITS><<<blocks, 256, 0, s>>>(o, L, rec64, theta, box, size,
-                                                     local, crit);
+                                                     local, crit, collapse);
         NBH_HIP(hipGetLastError());
     }
     if (between)
@@ -473,12 +497,13 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     const int M = o.n_nodes;
     if (mixed)
         link_sentinel_kernel<<<blocks_for(M, 256), 256, 0, s>>>(
-            w32link, make_int4(0, M, KIND_ERROR + 4, 0), M);
+            w32pair_link, make_int4(0x7F800000, M, KIND_ERROR + 4, 0), M, 2);
     else
         link_sentinel_kernel<<<blocks_for(M, 256), 256, 0, s>>>(
-            w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M);
+            w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M, 1);
     NBH_HIP(hipGetLastError());
-    const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32link, w32center};
+    const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32pair_link,
+                    w32center};
     for (int L = 0; L <= max_level; ++L) {
         if (mixed)
             level_down_kernel<true><<<blocks, 256, 0, s>>>(

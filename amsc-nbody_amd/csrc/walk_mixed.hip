@@ -10,10 +10,12 @@
 // Hot path.  Internal nodes, single-particle leaves and zero-mass nodes
 // share one straight-line path: "accept if d^2 > crit^2", with crit^2 = -1
 // for leaves and zero-mass nodes (always accepted; a zero mass adds exactly
-// zero, and with positive softening so does the target itself).  Both
-// 16-byte records of a node are fetched with scalar loads at the top of the
-// iteration.  Only multi-particle leaves, zero softening and error records
-// take the slow path.
+// zero, and with positive softening so does the target itself).  A node's
+// record is one 32-byte {node, link} pair fetched with a single scalar load
+// at the top of the iteration.  Only multi-particle leaves, zero softening
+// and error records take the slow path.  Single-child chains are collapsed
+// into their lowest member when the layout is built, which removes visits
+// without changing any lane's interaction list.
 //
 // Accumulation.  Each lane adds its interactions into FP32 partial sums and
 // folds them into a Kahan-compensated sum every kFold visits, so the
@@ -25,6 +27,13 @@ namespace nbh {
 namespace {
 
 constexpr int kFold = 16;
+
+// One position of the mixed walk: the node (fixed-point COM, G m) and its
+// link (crit^2, skip, kind | count << 8, first particle).
+struct alignas(32) WalkRec {
+    int4 node;
+    int4 link;
+};
 
 // Kahan-compensated FP32 accumulator of one component.
 struct Kahan {
@@ -81,7 +90,7 @@ __device__ __forceinline__ void interact(float dx, float dy, float dz, float d2,
 
 template <bool CONTAINS, bool SOFT, bool COUNT>
 __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
-    const int4* __restrict__ wnode, const int4* __restrict__ wlink,
+    const WalkRec* __restrict__ wrec,
     const int4* __restrict__ wcenter, const int* __restrict__ d_wcount,
     const int4* __restrict__ rec, const uint32_t* __restrict__ idx, int n,
     int nblocks_padded, float eps2, double* __restrict__ ax_out,
@@ -106,71 +115,78 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     int code = 0;
     int fold = 0;
     int i = 0;
+    // A lane's `next` is the first position it still has to see.  At
+    // position i a lane is active when next <= i.  Accepting (or finishing a
+    // leaf) sets next = skip; opening leaves next <= i, so the lane is active
+    // again at i + 1.  Lanes waiting for a later position always wait for
+    // skip[i] or beyond (subtrees nest), hence the wave moves to i + 1 when
+    // some lane opened and to skip[i] otherwise.
     while (i < wcount) {
-        const int4 lk = wlink[i];
-        const int4 nd = wnode[i];
+        const WalkRec r = wrec[i];
+        const int4 lk = r.link;
+        const int4 nd = r.node;
         const int skip = lk.y;
         const int kind = lk.z & 0xFF;
         const bool active = next <= i;
-        bool open = false;
         if (COUNT)
             ++visits;
-        if (kind == KIND_INTERNAL) {
-            float dx, dy, dz, d2;
-            separation(nd, tq, dx, dy, dz, d2);
-            bool acc = active && d2 > __int_as_float(lk.x);
-            if (CONTAINS && acc) {
-                const int4 c = wcenter[i];
-                const bool inside = abs(tq.x - c.x) <= c.w &&
-                                    abs(tq.y - c.y) <= c.w &&
-                                    abs(tq.z - c.z) <= c.w;
-                acc = !inside;
-            }
-            open = active && !acc;
-            if (__ballot(acc) != 0ull) {
-                interact(dx, dy, dz, d2, __int_as_float(nd.w), eps2, acc, a);
-                if (COUNT && acc && __int_as_float(nd.w) != 0.0f) {
-                    if ((lk.z >> 8) == 1) {
-                        if (lk.w != t)
-                            ++npp;
-                    } else {
-                        ++npn;
-                    }
-                }
-            }
-            next = acc ? skip : (open ? i + 1 : next);
-        } else if (kind == KIND_LEAF) {
-            // multi-particle leaf, or a single particle without softening
-            const int count = lk.z >> 8;
-            const int first = lk.w;
-            for (int k = first; k < first + count; ++k) {
-                const int4 s = (count == 1) ? nd : rec[k];
-                float dx, dy, dz, d2;
-                separation(s, tq, dx, dy, dz, d2);
-                const float gm = __int_as_float(s.w);
-                bool use = active && k != t && gm != 0.0f;
-                if (!SOFT && use && d2 == 0.0f) {
-                    code = 6;
-                    use = false;
-                }
-                interact(dx, dy, dz, d2, gm, eps2, use, a);
-                if (COUNT && use)
-                    ++npp;
-            }
-            next = active ? skip : next;
-        } else if (kind == KIND_SKIP) {
-            next = active ? skip : next;
-        } else if (active) {
-            code = (kind == KIND_CHILDLESS) ? 5 : kind - KIND_ERROR;
-            next = kIdle;
+        // Straight-line MAC for every record: records of other kinds carry
+        // crit^2 = +inf and are never accepted here.
+        float dx, dy, dz, d2;
+        separation(nd, tq, dx, dy, dz, d2);
+        bool acc = active && d2 > __int_as_float(lk.x);
+        if (CONTAINS && acc) {
+            const int4 c = wcenter[i];
+            const bool inside = abs(tq.x - c.x) <= c.w &&
+                                abs(tq.y - c.y) <= c.w &&
+                                abs(tq.z - c.z) <= c.w;
+            acc = !inside;
         }
-        if (code != 0)
-            next = kIdle;
+        if (__ballot(acc) != 0ull) {
+            interact(dx, dy, dz, d2, __int_as_float(nd.w), eps2, acc, a);
+            if (COUNT && acc && __int_as_float(nd.w) != 0.0f) {
+                if ((lk.z >> 8) == 1) {
+                    if (lk.w != t)
+                        ++npp;
+                } else {
+                    ++npn;
+                }
+            }
+        }
+        next = acc ? skip : next;
+        if (kind != KIND_INTERNAL) {
+            if (kind == KIND_LEAF) {
+                // multi-particle leaf, or a single particle without softening
+                const int count = lk.z >> 8;
+                const int first = lk.w;
+                for (int k = first; k < first + count; ++k) {
+                    const int4 s = rec[k];
+                    float sx, sy, sz, s2;
+                    separation(s, tq, sx, sy, sz, s2);
+                    const float gm = __int_as_float(s.w);
+                    bool use = active && k != t && gm != 0.0f;
+                    if (!SOFT && use && s2 == 0.0f) {
+                        code = 6;
+                        use = false;
+                    }
+                    interact(sx, sy, sz, s2, gm, eps2, use, a);
+                    if (COUNT && use)
+                        ++npp;
+                }
+                next = active ? skip : next;
+            } else if (kind == KIND_SKIP) {
+                next = active ? skip : next;
+            } else if (active) {
+                code = (kind == KIND_CHILDLESS) ? 5 : kind - KIND_ERROR;
+            }
+            if (code != 0)
+                next = kIdle;
+        }
         if (++fold == kFold) {
             a.fold();
             fold = 0;
         }
-        i = (__ballot(open) != 0ull) ? i + 1 : skip;
+        i = (__ballot(next <= i) != 0ull) ? i + 1 : skip;
     }
     a.fold();
     if (code != 0) {
@@ -205,7 +221,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
 
 int walk_blocks_padded(int n);
 
-void launch_walk_mixed(hipStream_t s, const int4* wnode, const int4* wlink,
+void launch_walk_mixed(hipStream_t s, const int4* wpairs,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx, int n,
                        float eps2, bool contains, bool soft, double* ax,
@@ -213,8 +229,9 @@ void launch_walk_mixed(hipStream_t s, const int4* wnode, const int4* wlink,
                        unsigned long long* counters)
 {
     const int nb = walk_blocks_padded(n);
+    const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
 #define NBH_ARGS                                                                 \
-    wnode, wlink, wcenter, d_wcount, rec, idx, n, nb, eps2, ax, ay, az, err,       \
+    wrec, wcenter, d_wcount, rec, idx, n, nb, eps2, ax, ay, az, err,       \
         counters
 #define NBH_LAUNCH(C, S, K)                                                      \
     walk_mixed_kernel<C, S, K><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS)

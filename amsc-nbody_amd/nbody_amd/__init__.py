@@ -74,6 +74,14 @@ DEFAULT_G = 6.67430e-11  # src/particle.hpp:16
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:
+    # torch carries its own HIP runtime with the same soname as the system
+    # one.  Loading torch first makes this library bind to that runtime, so
+    # the process holds exactly one HIP/HSA runtime; the other order loads
+    # two and the second one sees no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(path):
         raise ImportError(
             f"native library {path} is missing: build it with "
@@ -171,6 +179,7 @@ _sig("nbh_sim_phase_times", _i, _p, C.POINTER(_d), _i)
 _sig("nbh_sim_set_timing", _i, _p, _i)
 _sig("nbh_sim_count_interactions", _i, _p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64))
 _sig("nbh_generate", _i, _i, _sz, _u64, _p, _p, _p, _p, _p, _p, _p)
+_sig("nbh_copy_to_host", _i, _p, _p, _sz)
 _sig("nbh_direct_sum", _i, _p, _p, _p, _p, _p, _sz, _p, _sz, _d, _d, _p, _p, _p)
 
 
@@ -784,22 +793,12 @@ class Simulation:
         return pp.value, pn.value
 
 
-_hip = None
-
-
 def _copy_d2h(dptr, count, dtype):
     """Synchronous device-to-host copy of `count` elements at a raw pointer."""
-    global _hip
     out = np.empty(int(count), dtype=dtype)
     if count == 0:
         return out
-    if _hip is None:
-        _hip = C.CDLL("libamdhip64.so")
-        _hip.hipMemcpy.restype = C.c_int
-        _hip.hipMemcpy.argtypes = [_p, _p, _sz, C.c_int]
-    rc = _hip.hipMemcpy(out.ctypes.data, dptr, out.nbytes, 2)  # hipMemcpyDeviceToHost
-    if rc != 0:
-        raise DeviceError(f"hipMemcpy failed with {rc}", -6)
+    check(lib.nbh_copy_to_host(out.ctypes.data, dptr, out.nbytes))
     return out
 
 

@@ -177,6 +177,7 @@ def main():
                    "dt": args.dt, "precision": "mixed"},
         "steps_per_s": round(1e3 / ms_per_step, 4),
         "interactions_per_step": interactions,
+        "interactions_pp_pn": [pp, pn],
         "interactions_per_s": interactions * 1e3 / ms_per_step,
         "packet_efficiency": round(interactions / (64.0 * visits), 4) if visits else None,
         "fp64_value": None if fp64_rate is None else round(fp64_rate, 3),

@@ -58,6 +58,10 @@ typedef struct nbh_context nbh_context;
 int nbh_context_create(nbh_context** out, void* hip_stream);
 int nbh_context_destroy(nbh_context* ctx);
 
+/* Synchronous copy of `bytes` from device memory to host memory (waits for
+ * all work on the device first). */
+int nbh_copy_to_host(void* h_dst, const void* d_src, size_t bytes);
+
 /* --------------------------------------------------------- bounding box */
 /* Replaces Bbox::compute_box (gpu/cuda_kernels/globalbounding.cu:94-144).
  * Writes [centre_x, centre_y, centre_z, side] to d_box (device) and, when

@@ -196,3 +196,21 @@ def test_large_properties_and_direct_sum(nb, precision):
     pp, pn = sim.count_interactions()
     assert pp > 0 and pn > 0
     del v
+
+
+def test_binding_imported_before_torch_uses_one_runtime():
+    # A process that imports the binding first must still see the device
+    # (the binding loads torch's HIP runtime before the native library).
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import nbody_amd as nb\n"
+            "p = nb.generate('plummer', 4096, 1)\n"
+            "s = nb.Simulation(p, 1e-3, 0.5, 0.01, gravitational_constant=1.0)\n"
+            "s.initialize(); s.step()\n"
+            "print('ok', s.structure_host()['n_groups'])\n") % os.path.join(root, "amsc-nbody_amd")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.startswith("ok")
