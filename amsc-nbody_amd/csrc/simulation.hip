@@ -13,7 +13,7 @@
 // accelerations (fused with the bounding box), rebuild (Morton keys, sort,
 // groups, Karras, plan, octree, mass/COM, walk layout), forces into the next
 // buffer, closing kick, swap.
-#include "nbh_internal.hpp"
+#include "sim_internal.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -21,185 +21,13 @@
 
 namespace nbh {
 
-const char* traversal_message(int code);
-const char* stage_message(int word, int code);
-
-template <int BITS>
-void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
-                         bool mixed, const double4* rec64, const int4* rec32,
-                         double theta, const double* box, double G,
-                         bool need_center, bool soft, int* size, int* local,
-                         double* crit, int* pre, double4* w64node, double* w64crit,
-                         int4* w64link, double4* w64center, int4* w32node,
-                         int4* w32link, int4* w32center, int* wcount,
-                         hipEvent_t between);
-
 namespace {
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { reset(); }
-    void reset()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count)
-    {
-        reset();
-        if (count == 0)
-            count = 1;
-        NBH_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
-        n = count;
-    }
-    void ensure(size_t count)
-    {
-        if (count > n)
-            alloc(count + count / 4);
-    }
-};
-
-enum Phase : int {
-    PH_KICK_DRIFT_BBOX = 0,
-    PH_ENCODE,
-    PH_SORT,
-    PH_GATHER,
-    PH_GROUPS,
-    PH_KARRAS,
-    PH_PLAN,
-    PH_OCTREE,
-    PH_COM,
-    PH_WALK,
-    PH_TRAVERSAL,
-    PH_KICK,
-    PH_COUNT
-};
 
 const char* kPhaseNames[PH_COUNT] = {
     "kick_drift_bbox", "encode", "sort",  "gather",     "groups",    "karras",
     "plan",            "octree", "com",   "walk_layout", "traversal", "kick"};
 
-// Host copy of the mid-step readback (pinned).
-struct Readback {
-    int ngroups;
-    int nnodes;
-    int errors[ERR_WORD_COUNT];
-    int pad;
-    double box[4];
-};
-
 } // namespace
-} // namespace nbh
-
-struct nbh_sim {
-    nbh_sim_config cfg{};
-    size_t n = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    int state = 0; // 0 created, 1 ready, 2 failed
-    double time = 0.0;
-    size_t steps = 0;
-    bool timing = false;
-    int key_bytes = 8;
-    int passes = 8;
-
-    // particle state, original order
-    nbh::DevBuf<double> m, x, y, z, vx, vy, vz;
-    nbh::DevBuf<double> acc[2][3];
-    int cur = 0;
-
-    // build buffers sized for n
-    nbh::DevBuf<unsigned char> keys, keys_alt, unique;
-    nbh::DevBuf<uint32_t> idx, idx_alt;
-    nbh::DevBuf<double4> rec64;
-    nbh::DevBuf<int4> rec32;
-    nbh::DevBuf<int> gfirst, gcount;
-    nbh::DevBuf<int> left, right, tparent, rfirst, rlast, plen;
-    nbh::DevBuf<int> plevel, pcount, poffset;
-    // per-step cleared words: hist | sort status | group scan | plan scan |
-    // error words
-    nbh::DevBuf<uint32_t> zero;
-    size_t zero_words = 0;
-    uint32_t* hist = nullptr;
-    uint32_t* sort_status = nullptr;
-    unsigned long long* group_status = nullptr;
-    unsigned long long* plan_status = nullptr;
-    int* errors = nullptr;
-    // small device scalars
-    nbh::DevBuf<double> box;      // 4
-    nbh::DevBuf<double> partials; // bbox partials
-    nbh::DevBuf<int> scalars;     // [0] counter [1] ngroups [2] nnodes [3] wcount
-    nbh::DevBuf<unsigned long long> counters;
-
-    // octree (capacity grows)
-    size_t mcap = 0;
-    int nnodes = 0;
-    int ngroups = 0;
-    nbh::DevBuf<int> children, oparent, level, ofirst, ocount, pending,
-        child_count, size, local, pre;
-    nbh::DevBuf<unsigned char> prefix;
-    nbh::DevBuf<double> mass, comx, comy, comz, cenx, ceny, cenz, half, crit;
-    nbh::DevBuf<double4> w64node, w64center;
-    nbh::DevBuf<double> w64crit;
-    nbh::DevBuf<int4> w64link, w32node, w32center;
-
-    nbh::Readback* rb = nullptr;
-    hipEvent_t ev[nbh::PH_COUNT + 1] = {};
-    double phase_ms[nbh::PH_COUNT] = {};
-
-    ~nbh_sim()
-    {
-        if (rb)
-            (void)hipHostFree(rb);
-        for (auto& e : ev)
-            if (e)
-                (void)hipEventDestroy(e);
-        if (own_stream && stream)
-            (void)hipStreamDestroy(stream);
-    }
-
-    nbh_octree_view octree_view()
-    {
-        nbh_octree_view o{};
-        o.n_particles = static_cast<int>(n);
-        o.n_leaves = ngroups;
-        o.n_nodes = nnodes;
-        o.root = 0;
-        o.children = children.p;
-        o.parent = oparent.p;
-        o.level = level.p;
-        o.prefix = prefix.p;
-        o.first_particle = ofirst.p;
-        o.particle_count = ocount.p;
-        o.mass = mass.p;
-        o.com_x = comx.p;
-        o.com_y = comy.p;
-        o.com_z = comz.p;
-        o.center_x = cenx.p;
-        o.center_y = ceny.p;
-        o.center_z = cenz.p;
-        o.half_size = half.p;
-        o.pending_children = pending.p;
-        o.child_count = child_count.p;
-        return o;
-    }
-
-    void mark(int phase)
-    {
-        if (timing)
-            NBH_HIP(hipEventRecord(ev[phase], stream));
-    }
-};
-
-namespace nbh {
-namespace {
 
 bool mixed(const nbh_sim* s) { return s->cfg.precision == NBH_PRECISION_MIXED; }
 
@@ -437,6 +265,75 @@ void do_rebuild(nbh_sim* s, bool kick_drift)
         rebuild<21>(s, kick_drift);
 }
 
+void sim_initialize(nbh_sim* s)
+{
+    try {
+        do_rebuild(s, false);
+        forces(s, 1 - s->cur, nullptr);
+        s->mark(PH_KICK);
+        s->mark(PH_COUNT);
+        finish(s);
+        s->cur = 1 - s->cur;
+        s->state = 1;
+    } catch (...) {
+        s->state = 2;
+        throw;
+    }
+}
+
+void sim_step(nbh_sim* s)
+{
+    try {
+        do_rebuild(s, true);
+        const int next = 1 - s->cur;
+        forces(s, next, nullptr);
+        s->mark(PH_KICK);
+        auto& a = s->acc[next];
+        launch_kick(s->stream, s->vx.p, s->vy.p, s->vz.p, a[0].p, a[1].p,
+                    a[2].p, s->n, s->cfg.time_step);
+        s->mark(PH_COUNT);
+        finish(s);
+        s->cur = next;
+        s->time += s->cfg.time_step;
+        ++s->steps;
+    } catch (...) {
+        s->state = 2;
+        throw;
+    }
+}
+
+nbh_sim* sim_new(const nbh_sim_config* config, size_t count, void* hip_stream)
+{
+    validate_config(config);
+    if (count == 0)
+        fail(NBH_INVALID_ARGUMENT, "Simulation requires at least one particle");
+    if (count > static_cast<size_t>(std::numeric_limits<int>::max()) / 2 ||
+        count >= (1u << 30))
+        fail(NBH_OVERFLOW_ERROR, "Particle count exceeds the tree representation");
+    auto* s = new nbh_sim();
+    try {
+        s->cfg = *config;
+        s->n = count;
+        s->key_bytes = config->bits_per_axis == 10 ? 4 : 8;
+        s->passes = config->bits_per_axis == 10 ? Layout<10>::sort_passes
+                                                : Layout<21>::sort_passes;
+        if (hip_stream) {
+            s->stream = static_cast<hipStream_t>(hip_stream);
+        } else {
+            NBH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+            s->own_stream = true;
+        }
+        alloc_state(s);
+        for (auto& a : s->acc)
+            for (auto& c : a)
+                launch_fill_f64(s->stream, c.p, 0.0, count);
+    } catch (...) {
+        delete s;
+        throw;
+    }
+    return s;
+}
+
 void validate_config(const nbh_sim_config* c)
 {
     if (!c)
@@ -456,7 +353,6 @@ void validate_config(const nbh_sim_config* c)
         fail(NBH_INVALID_ARGUMENT, "bits_per_axis must be 10 or 21");
 }
 
-} // namespace
 } // namespace nbh
 
 using namespace nbh;
@@ -479,27 +375,10 @@ int nbh_sim_create(nbh_sim** out, const nbh_sim_config* config, size_t count,
         if (!out)
             fail(NBH_INVALID_ARGUMENT, "output pointer must not be null");
         validate_config(config);
-        if (count == 0)
-            fail(NBH_INVALID_ARGUMENT, "Simulation requires at least one particle");
-        if (count > static_cast<size_t>(std::numeric_limits<int>::max()) / 2 ||
-            count >= (1u << 30))
-            fail(NBH_OVERFLOW_ERROR, "Particle count exceeds the tree representation");
-        if (!h_mass || !h_x || !h_y || !h_z || !h_vx || !h_vy || !h_vz)
+        if (count != 0 && (!h_mass || !h_x || !h_y || !h_z || !h_vx || !h_vy || !h_vz))
             fail(NBH_INVALID_ARGUMENT, "particle arrays must not be null");
-        auto* s = new nbh_sim();
+        nbh_sim* s = sim_new(config, count, hip_stream);
         try {
-            s->cfg = *config;
-            s->n = count;
-            s->key_bytes = config->bits_per_axis == 10 ? 4 : 8;
-            s->passes = config->bits_per_axis == 10 ? Layout<10>::sort_passes
-                                                    : Layout<21>::sort_passes;
-            if (hip_stream) {
-                s->stream = static_cast<hipStream_t>(hip_stream);
-            } else {
-                NBH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-                s->own_stream = true;
-            }
-            alloc_state(s);
             const size_t bytes = count * sizeof(double);
             const double* src[7] = {h_mass, h_x, h_y, h_z, h_vx, h_vy, h_vz};
             nbh::DevBuf<double>* dst[7] = {&s->m, &s->x, &s->y, &s->z,
@@ -507,9 +386,6 @@ int nbh_sim_create(nbh_sim** out, const nbh_sim_config* config, size_t count,
             for (int k = 0; k < 7; ++k)
                 NBH_HIP(hipMemcpyAsync(dst[k]->p, src[k], bytes,
                                        hipMemcpyHostToDevice, s->stream));
-            for (auto& a : s->acc)
-                for (auto& c : a)
-                    launch_fill_f64(s->stream, c.p, 0.0, count);
             NBH_HIP(hipStreamSynchronize(s->stream));
         } catch (...) {
             delete s;
@@ -531,18 +407,7 @@ int nbh_sim_initialize(nbh_sim* s)
             fail(NBH_INVALID_ARGUMENT, "simulation must not be null");
         if (s->state != 0)
             fail(NBH_LOGIC_ERROR, "Simulation can only be initialized once");
-        try {
-            do_rebuild(s, false);
-            forces(s, 1 - s->cur, nullptr);
-            s->mark(PH_KICK);
-            s->mark(PH_COUNT);
-            finish(s);
-            s->cur = 1 - s->cur;
-            s->state = 1;
-        } catch (...) {
-            s->state = 2;
-            throw;
-        }
+        sim_initialize(s);
     });
 }
 
@@ -553,23 +418,7 @@ int nbh_sim_step(nbh_sim* s)
             fail(NBH_INVALID_ARGUMENT, "simulation must not be null");
         if (s->state != 1)
             fail(NBH_LOGIC_ERROR, "Simulation must be initialized before stepping");
-        try {
-            do_rebuild(s, true);
-            const int next = 1 - s->cur;
-            forces(s, next, nullptr);
-            s->mark(PH_KICK);
-            auto& a = s->acc[next];
-            launch_kick(s->stream, s->vx.p, s->vy.p, s->vz.p, a[0].p, a[1].p,
-                        a[2].p, s->n, s->cfg.time_step);
-            s->mark(PH_COUNT);
-            finish(s);
-            s->cur = next;
-            s->time += s->cfg.time_step;
-            ++s->steps;
-        } catch (...) {
-            s->state = 2;
-            throw;
-        }
+        sim_step(s);
     });
 }
 

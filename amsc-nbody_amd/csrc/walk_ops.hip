@@ -1,0 +1,410 @@
+// walk_ops.hip -- FP64 walks that apply another operation than the force to
+// the interaction list of each target: the softened potential (energy
+// reporting) and the force vector-Jacobian product (the adjoint).
+//
+// The walk is the one of the FP64 force traversal (same records, same
+// opening test, same list per target), so the potential and the gradient
+// belong to exactly the forces the step integrates.  Per interaction with a
+// source of mass m at separation d = x_s - x_t, r2 = |d|^2 + eps^2:
+//   a_t   += G m d r2^{-3/2}
+//   phi_t -= G m r2^{-1/2}
+//   g      = (da_t/dd)^T abar_t = G m (r2^{-3/2} abar - 3 r2^{-5/2} d (d . abar))
+//   xbar_t -= g,  xbar_s += g
+// A monopole source is a centre of mass, COM_A = sum_k m_k x_k / M_A, so its
+// cotangent reaches particle k as (m_k / M_A) xbar_A: node cotangents are
+// pushed down the octree, accumulating xbar_A / M_A from the root, and each
+// leaf particle adds m_k times the sum.  Source cotangents are reduced over
+// the wavefront (every lane sees the same source) before one atomic add.
+#include "walk_common.hpp"
+
+namespace nbh {
+
+namespace {
+
+enum : int { OP_POTENTIAL = 0, OP_VJP = 1 };
+
+struct OpArgs {
+    double* phi;                   // potential, original order
+    const double* abx;             // acceleration cotangent, original order
+    const double* aby;
+    const double* abz;
+    double* tbx;                   // target terms, Morton order (plain stores)
+    double* tby;
+    double* tbz;
+    double* pbx;                   // particle-source terms, Morton order
+    double* pby;
+    double* pbz;
+    double* nbx;                   // node-source terms, by walk position
+    double* nby;
+    double* nbz;
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+
+template <int OP>
+struct Lane {
+    double tx = 0.0, ty = 0.0, tz = 0.0;
+    double bx = 0.0, by = 0.0, bz = 0.0; // abar of the target
+    double phi = 0.0;
+    double gx = 0.0, gy = 0.0, gz = 0.0; // xbar of the target
+    int code = 0;
+
+    // One source for every lane of the wave (`use` false on lanes that do
+    // not interact).  Called from wave-uniform control flow.
+    __device__ __forceinline__ void source(bool use, double sx, double sy,
+                                           double sz, double sm, double G,
+                                           double eps2, double* dx_out,
+                                           double* dy_out, double* dz_out,
+                                           int index)
+    {
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        if (use && sm != 0.0) {
+            const double dx = sx - tx;
+            const double dy = sy - ty;
+            const double dz = sz - tz;
+            const double r2 = fma(dz, dz, fma(dx, dx, dy * dy)) + eps2;
+            if (r2 <= 0.0) {
+                code = 6;
+            } else {
+                const double inv = 1.0 / sqrt(r2);
+                const double gm = G * sm;
+                if (OP == OP_POTENTIAL) {
+                    phi = fma(-gm, inv, phi);
+                } else {
+                    const double inv3 = inv * inv * inv;
+                    const double dot = fma(dz, bz, fma(dx, bx, dy * by));
+                    const double c5 = 3.0 * dot * inv3 / r2;
+                    cx = gm * fma(inv3, bx, -c5 * dx);
+                    cy = gm * fma(inv3, by, -c5 * dy);
+                    cz = gm * fma(inv3, bz, -c5 * dz);
+                    gx -= cx;
+                    gy -= cy;
+                    gz -= cz;
+                }
+            }
+        }
+        if (OP == OP_VJP) {
+            if (__ballot(cx != 0.0 || cy != 0.0 || cz != 0.0) != 0ull) {
+                cx = wave_sum(cx);
+                cy = wave_sum(cy);
+                cz = wave_sum(cz);
+                if ((threadIdx.x & 63) == 0) {
+                    atomicAdd(dx_out + index, cx);
+                    atomicAdd(dy_out + index, cy);
+                    atomicAdd(dz_out + index, cz);
+                }
+            }
+        }
+    }
+};
+
+template <bool CONTAINS, int OP>
+__global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
+    const double4* __restrict__ wnode, const double* __restrict__ wcrit,
+    const int4* __restrict__ wlink, const double4* __restrict__ wcenter,
+    const int* __restrict__ d_wcount, const double4* __restrict__ rec,
+    const uint32_t* __restrict__ idx, int n, int nblocks_padded, double eps2,
+    double G, OpArgs a, int* err)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int packet = packet_of_block(nblocks_padded) * 4 + wave;
+    const int t = packet * 64 + lane;
+    if (packet * 64 >= n)
+        return;
+    const bool valid = t < n;
+    const int wcount = *d_wcount;
+    Lane<OP> L;
+    uint32_t o = 0;
+    if (valid) {
+        const double4 p = rec[t];
+        L.tx = p.x;
+        L.ty = p.y;
+        L.tz = p.z;
+        o = idx[t];
+        if (o >= static_cast<uint32_t>(n)) {
+            L.code = 3;
+            o = 0;
+        }
+        if (OP == OP_VJP) {
+            L.bx = a.abx[o];
+            L.by = a.aby[o];
+            L.bz = a.abz[o];
+        }
+    }
+    int next = valid ? 0 : 0x7FFFFFFF;
+    int i = 0;
+    while (i < wcount) {
+        const int4 lk = wlink[i];
+        const int skip = lk.x;
+        const int kind = lk.y;
+        const bool active = next <= i && L.code == 0;
+        bool open = false;
+        if (kind == KIND_INTERNAL || kind == KIND_CHILDLESS) {
+            const double4 nd = wnode[i];
+            const double dx = nd.x - L.tx;
+            const double dy = nd.y - L.ty;
+            const double dz = nd.z - L.tz;
+            const double d2 = fma(dz, dz, fma(dx, dx, dy * dy));
+            bool accept = active && sqrt(d2) > wcrit[i];
+            if (CONTAINS && accept) {
+                const double4 c = wcenter[i];
+                const bool inside = fabs(L.tx - c.x) <= c.w &&
+                                    fabs(L.ty - c.y) <= c.w &&
+                                    fabs(L.tz - c.z) <= c.w;
+                accept = !inside;
+            }
+            if (accept) {
+                next = skip;
+            } else if (active) {
+                if (kind == KIND_CHILDLESS) {
+                    L.code = 5;
+                } else {
+                    open = true;
+                    next = i + 1;
+                }
+            }
+            L.source(accept, nd.x, nd.y, nd.z, nd.w, G, eps2, a.nbx, a.nby,
+                     a.nbz, i);
+        } else if (kind == KIND_LEAF) {
+            const int first = lk.z;
+            const int count = lk.w;
+            if (count == 1) {
+                const double4 s = wnode[i];
+                L.source(active && first != t, s.x, s.y, s.z, s.w, G, eps2,
+                         a.pbx, a.pby, a.pbz, first);
+            } else {
+                for (int k = first; k < first + count; ++k) {
+                    const double4 s = rec[k];
+                    L.source(active && k != t, s.x, s.y, s.z, s.w, G, eps2,
+                             a.pbx, a.pby, a.pbz, k);
+                }
+            }
+            if (active)
+                next = skip;
+        } else if (kind == KIND_SKIP) {
+            if (active)
+                next = skip;
+        } else if (active) {
+            L.code = kind - KIND_ERROR;
+        }
+        i = (__ballot(open) != 0ull) ? i + 1 : skip;
+    }
+    if (L.code != 0) {
+        raise_error(err, L.code);
+        return;
+    }
+    if (!valid)
+        return;
+    if (OP == OP_POTENTIAL) {
+        a.phi[o] = L.phi;
+    } else {
+        a.tbx[t] = L.gx;
+        a.tby[t] = L.gy;
+        a.tbz[t] = L.gz;
+    }
+}
+
+// Top-down over one level: s_v = s_parent + nbar(pre_v) / M_v, then every
+// particle k of a leaf v adds m_k s_v to its source cotangent.
+__global__ __launch_bounds__(256) void push_down_kernel(
+    nbh_octree_view o, int L, const int* __restrict__ pre_of, int wcount,
+    const double* __restrict__ nbx, const double* __restrict__ nby,
+    const double* __restrict__ nbz, double* __restrict__ sx,
+    double* __restrict__ sy, double* __restrict__ sz,
+    const double4* __restrict__ rec, double* __restrict__ pbx,
+    double* __restrict__ pby, double* __restrict__ pbz)
+{
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+         v < o.n_nodes; v += stride) {
+        if (o.level[v] != L)
+            continue;
+        double ux = 0.0, uy = 0.0, uz = 0.0;
+        if (L > 0) {
+            const int p = o.parent[v];
+            ux = sx[p];
+            uy = sy[p];
+            uz = sz[p];
+        }
+        const int pre = pre_of[v];
+        const double m = o.mass[v];
+        if (m > 0.0 && pre >= 0 && pre < wcount) {
+            ux += nbx[pre] / m;
+            uy += nby[pre] / m;
+            uz += nbz[pre] / m;
+        }
+        sx[v] = ux;
+        sy[v] = uy;
+        sz[v] = uz;
+        const int pc = o.particle_count[v];
+        if (pc > 0) {
+            const int f = o.first_particle[v];
+            for (int k = f; k < f + pc; ++k) {
+                const double mk = rec[k].w;
+                pbx[k] = fma(mk, ux, pbx[k]);
+                pby[k] = fma(mk, uy, pby[k]);
+                pbz[k] = fma(mk, uz, pbz[k]);
+            }
+        }
+    }
+}
+
+// xbar (original order) += target terms + source terms of Morton slot t.
+__global__ __launch_bounds__(256) void combine_kernel(
+    const uint32_t* __restrict__ idx, int n, const double* __restrict__ tbx,
+    const double* __restrict__ tby, const double* __restrict__ tbz,
+    const double* __restrict__ pbx, const double* __restrict__ pby,
+    const double* __restrict__ pbz, double* __restrict__ gx,
+    double* __restrict__ gy, double* __restrict__ gz, int* err)
+{
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= n)
+        return;
+    const uint32_t o = idx[t];
+    if (o >= static_cast<uint32_t>(n)) {
+        raise_error(err, 3);
+        return;
+    }
+    gx[o] += tbx[t] + pbx[t];
+    gy[o] += tby[t] + pby[t];
+    gz[o] += tbz[t] + pbz[t];
+}
+
+// Per-block partial sums of 1/2 m v^2 and 1/2 m phi in a fixed order.
+__global__ __launch_bounds__(256) void energy_partials_kernel(
+    const double* __restrict__ m, const double* __restrict__ vx,
+    const double* __restrict__ vy, const double* __restrict__ vz,
+    const double* __restrict__ phi, size_t n, double* __restrict__ partials)
+{
+    __shared__ double sk[256];
+    __shared__ double sp[256];
+    double k = 0.0, p = 0.0;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         i < n; i += stride) {
+        const double v2 = fma(vz[i], vz[i], fma(vx[i], vx[i], vy[i] * vy[i]));
+        k = fma(0.5 * m[i], v2, k);
+        p = fma(0.5 * m[i], phi[i], p);
+    }
+    sk[threadIdx.x] = k;
+    sp[threadIdx.x] = p;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            sk[threadIdx.x] += sk[threadIdx.x + w];
+            sp[threadIdx.x] += sp[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sk[0];
+        partials[2 * blockIdx.x + 1] = sp[0];
+    }
+}
+
+template <int OP>
+void launch_op(hipStream_t s, const double4* wnode, const double* wcrit,
+               const int4* wlink, const double4* wcenter, const int* d_wcount,
+               const double4* rec, const uint32_t* idx, int n, double eps2,
+               double G, bool contains, const OpArgs& a, int* err)
+{
+    const int nb = walk_blocks_padded(n);
+    if (contains)
+        walk_op_kernel<true, OP><<<nb, kWalkThreads, 0, s>>>(
+            wnode, wcrit, wlink, wcenter, d_wcount, rec, idx, n, nb, eps2, G, a,
+            err);
+    else
+        walk_op_kernel<false, OP><<<nb, kWalkThreads, 0, s>>>(
+            wnode, wcrit, wlink, wcenter, d_wcount, rec, idx, n, nb, eps2, G, a,
+            err);
+    NBH_HIP(hipGetLastError());
+}
+
+} // namespace
+
+int walk_blocks_padded(int n);
+
+void launch_potential_fp64(hipStream_t s, const double4* wnode,
+                           const double* wcrit, const int4* wlink,
+                           const double4* wcenter, const int* d_wcount,
+                           const double4* rec, const uint32_t* idx, int n,
+                           double eps2, double G, bool contains, double* phi,
+                           int* err)
+{
+    OpArgs a{};
+    a.phi = phi;
+    launch_op<OP_POTENTIAL>(s, wnode, wcrit, wlink, wcenter, d_wcount, rec, idx,
+                            n, eps2, G, contains, a, err);
+}
+
+void launch_energy_partials(hipStream_t s, const double* m, const double* vx,
+                            const double* vy, const double* vz,
+                            const double* phi, size_t n, double* partials,
+                            int blocks)
+{
+    energy_partials_kernel<<<blocks, 256, 0, s>>>(m, vx, vy, vz, phi, n, partials);
+    NBH_HIP(hipGetLastError());
+}
+
+void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
+                           int max_level, const int* pre_of,
+                           const double4* wnode, const double* wcrit,
+                           const int4* wlink, const double4* wcenter,
+                           const int* d_wcount, int wcount_host,
+                           const double4* rec, const uint32_t* idx, int n,
+                           double eps2, double G, bool contains,
+                           const double* abx, const double* aby,
+                           const double* abz, double* scratch, double* gx,
+                           double* gy, double* gz, int* err)
+{
+    // scratch: tb (3n) | pb (3n) | nb (3 wcount) | s (3 n_nodes)
+    const size_t N = static_cast<size_t>(n);
+    const size_t W = static_cast<size_t>(wcount_host);
+    const size_t M = static_cast<size_t>(o.n_nodes);
+    double* tb = scratch;
+    double* pb = tb + 3 * N;
+    double* nbar = pb + 3 * N;
+    double* sv = nbar + 3 * W;
+    launch_fill_f64(s, pb, 0.0, 3 * N + 3 * W);
+    OpArgs a{};
+    a.abx = abx;
+    a.aby = aby;
+    a.abz = abz;
+    a.tbx = tb;
+    a.tby = tb + N;
+    a.tbz = tb + 2 * N;
+    a.pbx = pb;
+    a.pby = pb + N;
+    a.pbz = pb + 2 * N;
+    a.nbx = nbar;
+    a.nby = nbar + W;
+    a.nbz = nbar + 2 * W;
+    launch_op<OP_VJP>(s, wnode, wcrit, wlink, wcenter, d_wcount, rec, idx, n,
+                      eps2, G, contains, a, err);
+    unsigned blocks = blocks_for(M, 256);
+    if (blocks > 2048)
+        blocks = 2048;
+    for (int L = 0; L <= max_level; ++L) {
+        push_down_kernel<<<blocks, 256, 0, s>>>(
+            o, L, pre_of, wcount_host, a.nbx, a.nby, a.nbz, sv, sv + M,
+            sv + 2 * M, rec, a.pbx, a.pby, a.pbz);
+        NBH_HIP(hipGetLastError());
+    }
+    combine_kernel<<<blocks_for(N, 256), 256, 0, s>>>(idx, n, a.tbx, a.tby,
+                                                     a.tbz, a.pbx, a.pby, a.pbz,
+                                                     gx, gy, gz, err);
+    NBH_HIP(hipGetLastError());
+}
+
+size_t force_vjp_scratch_doubles(size_t n, size_t wcount, size_t n_nodes)
+{
+    return 6 * n + 3 * wcount + 3 * n_nodes;
+}
+
+} // namespace nbh

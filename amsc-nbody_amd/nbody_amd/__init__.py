@@ -181,6 +181,13 @@ _sig("nbh_sim_count_interactions", _i, _p, C.POINTER(_u64), C.POINTER(_u64), C.P
 _sig("nbh_generate", _i, _i, _sz, _u64, _p, _p, _p, _p, _p, _p, _p)
 _sig("nbh_copy_to_host", _i, _p, _p, _sz)
 _sig("nbh_direct_sum", _i, _p, _p, _p, _p, _p, _sz, _p, _sz, _d, _d, _p, _p, _p)
+_sig("nbh_sim_energy", _i, _p, C.POINTER(_d), C.POINTER(_d))
+_sig("nbh_sim_potential", _i, _p, _p)
+_sig("nbh_sim_force_vjp", _i, _p, _p, _p, _p, _p, _p, _p)
+_sig("nbh_adjoint_create", _i, C.POINTER(_p), C.POINTER(SimConfig), _sz, _i, _i, _p)
+_sig("nbh_adjoint_destroy", _i, _p)
+_sig("nbh_adjoint_forward", _i, _p, *([_p] * 13))
+_sig("nbh_adjoint_backward", _i, _p, *([_p] * 12))
 
 
 def check(status: int) -> None:
@@ -791,6 +798,100 @@ class Simulation:
         if with_visits:
             return pp.value, pn.value, wv.value
         return pp.value, pn.value
+
+    def energy(self):
+        """(kinetic, potential) of the current state; the potential walks the
+        current tree in FP64 (FP64 simulations only)."""
+        k, u = _d(0.0), _d(0.0)
+        check(lib.nbh_sim_energy(self.handle, C.byref(k), C.byref(u)))
+        return k.value, u.value
+
+    def potential(self):
+        """Per-particle potential (device tensor, original order)."""
+        phi = empty(self.count, _torch().float64)
+        check(lib.nbh_sim_potential(self.handle, ptr(phi)))
+        return phi
+
+    def force_vjp(self, abx, aby, abz):
+        """(da/dx)^T abar at the current tree; device float64 tensors in,
+        device tensors out (FP64 simulations only)."""
+        torch = _torch()
+        ab = [t.to(device=device(), dtype=torch.float64).contiguous() for t in (abx, aby, abz)]
+        g = [empty(self.count, torch.float64) for _ in range(3)]
+        check(lib.nbh_sim_force_vjp(self.handle, *[ptr(t) for t in ab], *[ptr(t) for t in g]))
+        return g
+
+
+# ----------------------------------------------------------------- adjoint
+class Adjoint:
+    """Reverse mode of `steps` KDK steps of the FP64 Barnes-Hut pipeline with
+    checkpoints every `checkpoint_every` steps (device tensors in and out)."""
+
+    def __init__(self, count: int, time_step: float, theta: float, softening: float,
+                 steps: int, checkpoint_every: int, gravitational_constant: float = DEFAULT_G,
+                 bits_per_axis=21, stream=None):
+        device()
+        self.config = SimConfig(time_step, theta, softening, gravitational_constant,
+                                PRECISION_FP64, bits_per_axis)
+        self.count = count
+        self.handle = _p()
+        check(lib.nbh_adjoint_create(C.byref(self.handle), C.byref(self.config), count,
+                                     steps, checkpoint_every, stream))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.nbh_adjoint_destroy(self.handle)
+            self.handle = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, m, x, y, z, vx, vy, vz):
+        torch = _torch()
+        ins = [t.to(device=device(), dtype=torch.float64).contiguous()
+               for t in (m, x, y, z, vx, vy, vz)]
+        outs = [empty(self.count, torch.float64) for _ in range(6)]
+        check(lib.nbh_adjoint_forward(self.handle, *[ptr(t) for t in ins],
+                                      *[ptr(t) for t in outs]))
+        return outs
+
+    def backward(self, xb, yb, zb, vxb, vyb, vzb):
+        torch = _torch()
+        ins = [t.to(device=device(), dtype=torch.float64).contiguous()
+               for t in (xb, yb, zb, vxb, vyb, vzb)]
+        outs = [empty(self.count, torch.float64) for _ in range(6)]
+        check(lib.nbh_adjoint_backward(self.handle, *[ptr(t) for t in ins],
+                                       *[ptr(t) for t in outs]))
+        return outs
+
+
+def leapfrog_function():
+    """torch.autograd.Function over Adjoint: (m, pos[3,N], vel[3,N]) ->
+    (pos_K, vel_K); gradients reach pos and vel (masses are constants).
+    For checking the adjoint against finite differences."""
+    torch = _torch()
+
+    class LeapfrogBH(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, adjoint, m, pos, vel):
+            out = adjoint.forward(m, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2])
+            ctx.adjoint = adjoint
+            return torch.stack(out[:3]), torch.stack(out[3:])
+
+        @staticmethod
+        def backward(ctx, gpos, gvel):
+            torch_ = _torch()
+            if gpos is None:
+                gpos = torch_.zeros(3, ctx.adjoint.count, dtype=torch_.float64, device=device())
+            if gvel is None:
+                gvel = torch_.zeros(3, ctx.adjoint.count, dtype=torch_.float64, device=device())
+            g = ctx.adjoint.backward(gpos[0], gpos[1], gpos[2], gvel[0], gvel[1], gvel[2])
+            return None, None, torch_.stack(g[:3]), torch_.stack(g[3:])
+
+    return LeapfrogBH
 
 
 def _copy_d2h(dptr, count, dtype):

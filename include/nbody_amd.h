@@ -299,6 +299,48 @@ int nbh_sim_set_timing(nbh_sim* sim, int enabled);
 int nbh_sim_count_interactions(nbh_sim* sim, uint64_t* pp, uint64_t* pn,
                                uint64_t* wave_visits);
 
+/* ------------------------------------------------------- energy, adjoint */
+/* No reference entry point: the reference evaluates energies only inside its
+ * tests (tests/unit/cuda/simulation_test.cu, two-body orbit). Both functions
+ * walk the current tree in FP64 (precision NBH_PRECISION_FP64 only): the
+ * potential of particle i is -sum G m_j (r^2 + eps^2)^{-1/2} over the same
+ * interaction list as its force.  kinetic = 1/2 sum m v^2,
+ * potential = 1/2 sum m phi.  d_phi: device array of count, original order. */
+int nbh_sim_energy(nbh_sim* sim, double* kinetic, double* potential);
+int nbh_sim_potential(nbh_sim* sim, double* d_phi);
+
+/* Force vector-Jacobian product at the current tree of an FP64 simulation:
+ * g = (da/dx)^T abar, the tree (interaction lists) held fixed.  All arrays
+ * are device arrays of count in original order.  Pairs with
+ * nbh_sim_accelerations as the forward. */
+int nbh_sim_force_vjp(nbh_sim* sim, const double* d_abx, const double* d_aby,
+                      const double* d_abz, double* d_gx, double* d_gy,
+                      double* d_gz);
+
+/* Reverse mode of `steps` kick-drift-kick steps (FP64 pipeline, tree fixed
+ * within each force evaluation).  Forward takes the initial state (device
+ * arrays, original order), stores (x, v) every checkpoint_every steps and
+ * returns the final state; backward takes the cotangents of the final
+ * positions and velocities and returns those of the initial ones, recomputing
+ * each segment from its checkpoint.  Masses are not differentiated. */
+typedef struct nbh_adjoint nbh_adjoint;
+int nbh_adjoint_create(nbh_adjoint** out, const nbh_sim_config* config,
+                       size_t count, int steps, int checkpoint_every,
+                       void* hip_stream);
+int nbh_adjoint_destroy(nbh_adjoint* adjoint);
+int nbh_adjoint_forward(nbh_adjoint* adjoint, const double* d_mass,
+                        const double* d_x, const double* d_y, const double* d_z,
+                        const double* d_vx, const double* d_vy,
+                        const double* d_vz, double* d_x_out, double* d_y_out,
+                        double* d_z_out, double* d_vx_out, double* d_vy_out,
+                        double* d_vz_out);
+int nbh_adjoint_backward(nbh_adjoint* adjoint, const double* d_xbar,
+                         const double* d_ybar, const double* d_zbar,
+                         const double* d_vxbar, const double* d_vybar,
+                         const double* d_vzbar, double* d_x0bar,
+                         double* d_y0bar, double* d_z0bar, double* d_vx0bar,
+                         double* d_vy0bar, double* d_vz0bar);
+
 /* ------------------------------------------------------------ workloads */
 /* Seeded synthetic particle sets (host arrays of length count).
  * kind: 0 Plummer (a=1, M=1, truncated at 10a, equal masses)

@@ -214,3 +214,78 @@ def single_particle_tree():
         mass=np.array([2.0]), com_x=np.array([3.0]), com_y=np.array([4.0]),
         com_z=np.array([5.0]), center_x=np.array([3.0]), center_y=np.array([4.0]),
         center_z=np.array([5.0]), half=np.array([1.0]))
+
+
+# ------------------------------------------------------ direct-sum adjoint
+def direct_potential_numpy(m, x, y, z, eps, G=1.0):
+    """Softened potential phi_i = -sum_j G m_j (r^2 + eps^2)^{-1/2}, j != i."""
+    dx = x[None, :] - x[:, None]
+    dy = y[None, :] - y[:, None]
+    dz = z[None, :] - z[:, None]
+    r2 = dx * dx + dy * dy + dz * dz + eps * eps
+    np.fill_diagonal(r2, np.inf)
+    return -(G * m[None, :] / np.sqrt(r2)).sum(1)
+
+
+def direct_vjp_numpy(m, pos, abar, eps, G=1.0):
+    """(da/dx)^T abar of the softened direct sum.  pos, abar: (3, N).
+
+    a_i = sum_j G m_j d_ij r_ij^-3 with d_ij = x_j - x_i; the Jacobian block
+    J_ij = G m_j (r^-3 I - 3 r^-5 d d^T) is symmetric, so xbar_j += J_ij abar_i
+    and xbar_i -= J_ij abar_i.  Written from the closed form, independently of
+    the tree code."""
+    d = pos[:, None, :] - pos[:, :, None]          # (3, i, j) = x_j - x_i
+    r2 = (d * d).sum(0) + eps * eps
+    np.fill_diagonal(r2, np.inf)
+    inv3 = r2 ** -1.5
+    inv5 = inv3 / r2
+    gm = G * m[None, :]
+    dot = (d * abar[:, :, None]).sum(0)            # d_ij . abar_i
+    g = gm[None] * (inv3[None] * abar[:, :, None] - 3.0 * inv5[None] * dot[None] * d)
+    return g.sum(1) - g.sum(2)                     # sources gain, targets lose
+
+
+def leapfrog_numpy(m, pos, vel, dt, steps, eps, G=1.0):
+    """KDK with the direct sum; returns the positions x_0..x_K and the final
+    velocities."""
+    def acc(p):
+        return np.stack(direct_numpy(m, p[0], p[1], p[2], eps, G))
+    xs = [pos.copy()]
+    a = acc(pos)
+    x, v = pos.copy(), vel.copy()
+    for _ in range(steps):
+        v = v + 0.5 * dt * a
+        x = x + dt * v
+        a = acc(x)
+        v = v + 0.5 * dt * a
+        xs.append(x.copy())
+    return xs, v
+
+
+def leapfrog_adjoint_numpy(m, pos, vel, dt, steps, eps, xbar, vbar, G=1.0):
+    """Reverse sweep of leapfrog_numpy: cotangents of (x_0, v_0) from those
+    of (x_K, v_K)."""
+    xs, _ = leapfrog_numpy(m, pos, vel, dt, steps, eps, G)
+    xb, vb = xbar.copy(), vbar.copy()
+    for j in range(steps, 0, -1):
+        ab = (0.5 * dt if j == steps else dt) * vb
+        xb = xb + direct_vjp_numpy(m, xs[j], ab, eps, G)
+        vb = vb + dt * xb
+    xb = xb + direct_vjp_numpy(m, xs[0], 0.5 * dt * vb, eps, G)
+    return xb, vb
+
+
+def truncated_plummer_energy(rmax=10.0, G=1.0):
+    """Potential energy of a unit-mass Plummer sphere (a = 1) truncated at
+    rmax and renormalised: W = -G int_0^R M(r) M'(r) / r dr with
+    M(r) = r^3 (1 + r^2)^{-3/2} / M(R).  For R -> inf this is -3 pi / 32."""
+    from scipy.integrate import quad
+    mr = rmax ** 3 / (1.0 + rmax ** 2) ** 1.5
+
+    def integrand(r):
+        mass = r ** 3 / (1.0 + r * r) ** 1.5 / mr
+        dm = 3.0 * r * r / (1.0 + r * r) ** 2.5 / mr
+        return mass * dm / r
+
+    val, _ = quad(integrand, 0.0, rmax, limit=200)
+    return -G * val
