@@ -8,9 +8,13 @@ Simulation::step (src/simulation.cpp:100-119) -- with inputs resident in HBM.
 
 Prints one JSON line: Mparticle-steps/s of the mixed-FP32 pipeline, the FP64
 pipeline's rate and the speed-up, the accuracy of the mixed forces against
-the FP64 pipeline and an FP64 direct sum, momentum/energy drift, the
+the FP64 pipeline and an FP64 direct sum (at theta 0.3 / 0.5 / 0.8 for the
+Plummer workload), momentum and energy drift over the timed steps, the
 traversal's achieved VALU FLOP/s against the MI355X FP32 peak, and the CPU
 oracle timed on a bounded sample of the same workload.
+
+Energy: kinetic + potential, the potential from an FP64 walk over a tree
+built at the state (Simulation.energy of an FP64 simulation of that state).
 """
 
 import argparse
@@ -59,6 +63,53 @@ def time_steps(nb, torch, sim, steps):
     return time.perf_counter() - t0, phases
 
 
+def traversal_traffic():
+    """HBM bytes per traversal dispatch from the committed counter summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, calibrated against a copy of
+    known size), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "walk_mixed_plummer4m_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_dispatch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def energy_of(nb, state, args):
+    """(E, K, U) of a state: an FP64 simulation of it, initialised (tree and
+    forces), then its FP64 potential walk."""
+    s = nb.Simulation(state, args.dt, args.theta, args.softening, gravitational_constant=1.0,
+                      precision=nb.PRECISION_FP64, bits_per_axis=21)
+    s.initialize()
+    k, u = s.energy()
+    s.close()
+    return k + u, k, u
+
+
+def accuracy_at(nb, torch, p, theta, args, targets, exact):
+    """Mixed vs FP64 Barnes-Hut and vs the direct sum at one theta, with the
+    traversal time of each precision (one force evaluation)."""
+    out = {}
+    acc = {}
+    for name, prec in (("mixed", nb.PRECISION_MIXED), ("fp64", nb.PRECISION_FP64)):
+        s = nb.Simulation(p, args.dt, theta, args.softening, gravitational_constant=1.0,
+                          precision=prec, bits_per_axis=21)
+        s.set_timing(True)
+        s.initialize()
+        acc[name] = s.accelerations()
+        out[f"{name}_traversal_ms"] = round(s.phase_times()["traversal"], 3)
+        s.close()
+    r, mx = rel_err(acc["mixed"], acc["fp64"])
+    out["rms_vs_fp64_bh"], out["max_vs_fp64_bh"] = float(f"{r:.4g}"), float(f"{mx:.4g}")
+    r, mx = rel_err([a[targets] for a in acc["mixed"]], exact)
+    out["rms_vs_direct"], out["max_vs_direct"] = float(f"{r:.4g}"), float(f"{mx:.4g}")
+    r, _ = rel_err([a[targets] for a in acc["fp64"]], exact)
+    out["fp64_rms_vs_direct"] = float(f"{r:.4g}")
+    return out
+
+
 def cpu_baseline(p, args):
     """The oracle (the reference algorithm in FP64 C++/OpenMP) on the same
     particles: one full rebuild plus the forces of a bounded sample of
@@ -93,7 +144,12 @@ def main():
     ap.add_argument("--workload", default="plummer")
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--direct-targets", type=int, default=8192)
-    ap.add_argument("--cpu-targets", type=int, default=32768)
+    ap.add_argument("--cpu-targets", type=int, default=524288)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this size first (the "
+                         "reference dispatch for FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--thetas", default="0.3,0.8",
+                    help="extra opening angles for the accuracy table (Plummer only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
@@ -104,6 +160,14 @@ def main():
 
     import torch
     import nbody_amd as nb
+
+    if args.calibrate_copy_mib > 0:
+        words = args.calibrate_copy_mib * (1 << 20) // 8
+        src = torch.ones(words, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     n = args.n
     p = nb.generate(args.workload, n, 42)
@@ -142,19 +206,30 @@ def main():
 
     s0 = mixed.state()
     p0 = momentum(s0)
+    e0 = energy_of(nb, s0, args)
     for _ in range(args.warmup):
         mixed.step()
     secs, phases = time_steps(nb, torch, mixed, args.steps)
     s1 = mixed.state()
     p1 = momentum(s1)
+    e1 = energy_of(nb, s1, args)
     ms_per_step = 1e3 * secs / args.steps
     value = n * args.steps / secs / 1e6
 
+    energy64 = None
     if not args.no_fp64:
         f64.step()  # warm-up
+        f0 = f64.energy()
         secs64, phases64 = time_steps(nb, torch, f64, args.fp64_steps)
         fp64_rate = n * args.fp64_steps / secs64 / 1e6
+        f1 = f64.energy()
+        energy64 = (sum(f1) - sum(f0)) / abs(sum(f0))
         f64.close()
+
+    by_theta = {}
+    if args.workload == "plummer" and args.thetas:
+        for th in (float(t) for t in args.thetas.split(",")):
+            by_theta[str(th)] = accuracy_at(nb, torch, p, th, args, targets, exact)
 
     trav_ms = phases["traversal"]
     achieved = FLOPS_PER_INTERACTION * interactions / (trav_ms * 1e-3) / 1e12
@@ -184,10 +259,19 @@ def main():
         "speedup_vs_fp64": None if fp64_rate is None else round(value / fp64_rate, 3),
         "accuracy": {k: float(f"{v:.4g}") for k, v in acc.items()},
         "momentum_drift": float(np.abs(p1 - p0).max()),
+        "energy": {"E_start": e0[0], "E_end": e1[0], "kinetic_start": e0[1],
+                   "potential_start": e0[2],
+                   "rel_drift": (e1[0] - e0[0]) / abs(e0[0]),
+                   "steps": args.warmup + args.steps,
+                   "fp64_rel_drift": energy64, "fp64_steps": args.fp64_steps},
+        "accuracy_by_theta": by_theta,
         "phase_ms": {k: round(v, 4) for k, v in phases.items()},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),
+                     "traffic": traversal_traffic() if (args.workload == "plummer" and
+                                                        n == 4_194_304 and
+                                                        args.theta == 0.5) else None,
                      "kernel": "walk_mixed_kernel",
                      "flops_per_interaction": FLOPS_PER_INTERACTION},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(p, args),
