@@ -153,7 +153,8 @@ __device__ __forceinline__ void finish_box(const double* partials,
 __global__ __launch_bounds__(kBoxThreads) void bbox_kernel(
     const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, size_t n, double* __restrict__ partials,
-    unsigned* counter, double* box, int* err)
+    unsigned* counter, double* box, int* err, const double* __restrict__ m,
+    double4* __restrict__ pos4)
 {
     const double inf = __builtin_inf();
     Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false};
@@ -161,6 +162,8 @@ __global__ __launch_bounds__(kBoxThreads) void bbox_kernel(
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
          i < n; i += stride) {
         const double px = x[i], py = y[i], pz = z[i];
+        if (pos4)
+            pos4[i] = make_double4(px, py, pz, m[i]);
         e.bad = e.bad || !(isfinite(px) && isfinite(py) && isfinite(pz));
         e.lo[0] = fmin(e.lo[0], px);
         e.lo[1] = fmin(e.lo[1], py);
@@ -181,7 +184,8 @@ __global__ __launch_bounds__(kBoxThreads) void kick_drift_bbox_kernel(
     double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
     const double* __restrict__ ax, const double* __restrict__ ay,
     const double* __restrict__ az, size_t n, double dt,
-    double* __restrict__ partials, unsigned* counter, double* box, int* err)
+    double* __restrict__ partials, unsigned* counter, double* box, int* err,
+    const double* __restrict__ m, double4* __restrict__ pos4)
 {
     const double half = 0.5 * dt;
     const double inf = __builtin_inf();
@@ -201,6 +205,8 @@ __global__ __launch_bounds__(kBoxThreads) void kick_drift_bbox_kernel(
         x[i] = px;
         y[i] = py;
         z[i] = pz;
+        if (pos4)
+            pos4[i] = make_double4(px, py, pz, m[i]);
         e.bad = e.bad || !(isfinite(px) && isfinite(py) && isfinite(pz));
         e.lo[0] = fmin(e.lo[0], px);
         e.lo[1] = fmin(e.lo[1], py);
@@ -314,7 +320,8 @@ __global__ __launch_bounds__(256) void gather_kernel(
     const uint32_t* __restrict__ idx, const double* __restrict__ m,
     const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, size_t n, double4* __restrict__ rec64,
-    int4* __restrict__ rec32, const double* __restrict__ box, double G)
+    int4* __restrict__ rec32, const double* __restrict__ box, double G,
+    const double4* __restrict__ pos4)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -322,7 +329,19 @@ __global__ __launch_bounds__(256) void gather_kernel(
     uint32_t p = idx[i];
     if (p >= n)
         p = 0; // unreachable for a permutation; keeps every read in bounds
-    const double px = x[p], py = y[p], pz = z[p], pm = m[p];
+    double px, py, pz, pm;
+    if (pos4) {
+        const double4 q = pos4[p]; // one 32-byte read instead of four
+        px = q.x;
+        py = q.y;
+        pz = q.z;
+        pm = q.w;
+    } else {
+        px = x[p];
+        py = y[p];
+        pz = z[p];
+        pm = m[p];
+    }
     rec64[i] = make_double4(px, py, pz, pm);
     if (rec32) {
         const double side = box[3];
@@ -343,10 +362,10 @@ __global__ __launch_bounds__(256) void gather_kernel(
 
 void launch_bbox(nbh_context* ctx, const double* x, const double* y,
                  const double* z, size_t n, double* d_box, double* partials,
-                 unsigned* counter, int* err)
+                 unsigned* counter, int* err, const double* m, double4* pos4)
 {
     bbox_kernel<<<box_blocks(n), kBoxThreads, 0, ctx->stream>>>(
-        x, y, z, n, partials, counter, d_box, err);
+        x, y, z, n, partials, counter, d_box, err, m, pos4);
     NBH_HIP(hipGetLastError());
 }
 
@@ -355,11 +374,12 @@ void launch_kick_drift_bbox(hipStream_t s, double* x, double* y, double* z,
                             const double* ax, const double* ay,
                             const double* az, size_t n, double dt,
                             double* d_box, double* partials,
-                            unsigned* counter, int* err)
+                            unsigned* counter, int* err, const double* m,
+                            double4* pos4)
 {
     kick_drift_bbox_kernel<<<box_blocks(n), kBoxThreads, 0, s>>>(
         x, y, z, vx, vy, vz, ax, ay, az, n, dt, partials, counter, d_box,
-        err);
+        err, m, pos4);
     NBH_HIP(hipGetLastError());
 }
 
@@ -389,10 +409,10 @@ template void launch_encode<21>(hipStream_t, const double*, const double*,
 void launch_gather(hipStream_t s, const uint32_t* idx, const double* m,
                    const double* x, const double* y, const double* z,
                    size_t n, double4* rec64, int4* rec32, const double* box,
-                   double G)
+                   double G, const double4* pos4)
 {
     gather_kernel<<<blocks_for(n, 256), 256, 0, s>>>(idx, m, x, y, z, n, rec64,
-                                                     rec32, box, G);
+                                                     rec32, box, G, pos4);
     NBH_HIP(hipGetLastError());
 }
 

@@ -160,24 +160,30 @@ void launch_fill_u64(hipStream_t s, unsigned long long* p,
 
 // bbox_morton.hip
 size_t bbox_partial_doubles();
+// With m and pos4 given, both also write the packed {x, y, z, m} copy of
+// every particle (original order) that the Morton gather reads.
 void launch_bbox(nbh_context* ctx, const double* x, const double* y,
                  const double* z, size_t n, double* d_box, double* partials,
-                 unsigned* counter, int* err);
+                 unsigned* counter, int* err, const double* m = nullptr,
+                 double4* pos4 = nullptr);
 void launch_kick_drift_bbox(hipStream_t s, double* x, double* y, double* z,
                             double* vx, double* vy, double* vz,
                             const double* ax, const double* ay,
                             const double* az, size_t n, double dt,
                             double* d_box, double* partials,
-                            unsigned* counter, int* err);
+                            unsigned* counter, int* err,
+                            const double* m = nullptr, double4* pos4 = nullptr);
 template <int BITS>
 void launch_encode(hipStream_t s, const double* x, const double* y,
                    const double* z, size_t n, const double* d_box,
                    typename Layout<BITS>::Key* keys, uint32_t* idx,
                    uint32_t* hist /* passes*256, zeroed */);
+// pos4 (optional): packed {x, y, z, m} in original order, read instead of
+// the four planar arrays.
 void launch_gather(hipStream_t s, const uint32_t* idx, const double* m,
                    const double* x, const double* y, const double* z,
                    size_t n, double4* rec64, int4* rec32, const double* box,
-                   double G);
+                   double G, const double4* pos4 = nullptr);
 
 // radix_sort.hip
 size_t sort_status_words(size_t n, int passes);

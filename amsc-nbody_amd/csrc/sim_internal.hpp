@@ -14,10 +14,13 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          bool mixed, const double4* rec64, const int4* rec32,
                          double theta, const double* box, double G,
                          bool need_center, bool soft, int* size, int* local,
-                         double* crit, int* pre, double4* w64node, double* w64crit,
+                         double* crit, int* pre, int* order, int* level_info,
+                         double4* w64node, double* w64crit,
                          int4* w64link, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
                          hipEvent_t between);
+// ints of the level-pass bookkeeping array (level starts + per-block counts)
+size_t level_info_ints();
 
 template <typename T>
 struct DevBuf {
@@ -99,6 +102,7 @@ struct nbh_sim {
     nbh::DevBuf<unsigned char> keys, keys_alt, unique;
     nbh::DevBuf<uint32_t> idx, idx_alt;
     nbh::DevBuf<double4> rec64;
+    nbh::DevBuf<double4> pos4; // packed {x, y, z, m}, original order
     nbh::DevBuf<int4> rec32;
     nbh::DevBuf<int> gfirst, gcount;
     nbh::DevBuf<int> left, right, tparent, rfirst, rlast, plen;
@@ -123,7 +127,7 @@ struct nbh_sim {
     int nnodes = 0;
     int ngroups = 0;
     nbh::DevBuf<int> children, oparent, level, ofirst, ocount, pending,
-        child_count, size, local, pre;
+        child_count, size, local, pre, order, level_info;
     nbh::DevBuf<unsigned char> prefix;
     nbh::DevBuf<double> mass, comx, comy, comz, cenx, ceny, cenz, half, crit;
     nbh::DevBuf<double4> w64node, w64center;

@@ -45,6 +45,7 @@ void alloc_state(nbh_sim* s)
     s->idx.alloc(n);
     s->idx_alt.alloc(n);
     s->rec64.alloc(n);
+    s->pos4.alloc(n);
     if (mixed(s))
         s->rec32.alloc(n);
     s->gfirst.alloc(n);
@@ -96,8 +97,9 @@ void ensure_octree(nbh_sim* s, size_t m)
     const size_t cap = m + m / 4 + 1024;
     s->children.alloc(8 * cap);
     for (auto* b : {&s->oparent, &s->level, &s->ofirst, &s->ocount, &s->pending,
-                    &s->child_count, &s->size, &s->local, &s->pre})
+                    &s->child_count, &s->size, &s->local, &s->pre, &s->order})
         b->alloc(cap);
+    s->level_info.alloc(level_info_ints());
     s->prefix.alloc(cap * s->key_bytes);
     for (auto* b : {&s->mass, &s->comx, &s->comy, &s->comz, &s->cenx, &s->ceny,
                     &s->cenz, &s->half, &s->crit})
@@ -133,12 +135,12 @@ void rebuild(nbh_sim* s, bool kick_drift)
         launch_kick_drift_bbox(st, s->x.p, s->y.p, s->z.p, s->vx.p, s->vy.p,
                                s->vz.p, c[0].p, c[1].p, c[2].p, n,
                                s->cfg.time_step, s->box.p, s->partials.p,
-                               counter, s->errors + ERR_BBOX);
+                               counter, s->errors + ERR_BBOX, s->m.p, s->pos4.p);
     } else {
         nbh_context tmp;
         tmp.stream = st;
         launch_bbox(&tmp, s->x.p, s->y.p, s->z.p, n, s->box.p, s->partials.p,
-                    counter, s->errors + ERR_BBOX);
+                    counter, s->errors + ERR_BBOX, s->m.p, s->pos4.p);
     }
     s->mark(PH_ENCODE);
     K* keys = reinterpret_cast<K*>(s->keys.p);
@@ -150,7 +152,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
     s->mark(PH_GATHER);
     launch_gather(st, s->idx.p, s->m.p, s->x.p, s->y.p, s->z.p, n, s->rec64.p,
                   mixed(s) ? s->rec32.p : nullptr, s->box.p,
-                  s->cfg.gravitational_constant);
+                  s->cfg.gravitational_constant, s->pos4.p);
     s->mark(PH_GROUPS);
     int* d_ng = s->scalars.p + 1;
     int* d_nm = s->scalars.p + 2;
@@ -202,7 +204,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
     launch_level_passes<BITS>(
         st, o, Layout<BITS>::bits, mixed(s), s->rec64.p, s->rec32.p,
         s->cfg.theta, s->box.p, s->cfg.gravitational_constant, contains,
-        s->cfg.softening > 0.0, s->size.p, s->local.p, s->crit.p, s->pre.p, s->w64node.p, s->w64crit.p,
+        s->cfg.softening > 0.0, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->w64node.p, s->w64crit.p,
         s->w64link.p, s->w64center.p, s->w32node.p, nullptr,
         s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr);
 }

@@ -287,16 +287,17 @@ template <int BITS>
 __global__ __launch_bounds__(256) void level_up_kernel(
     nbh_octree_view o, int L, const double4* __restrict__ rec, double theta,
     const double* __restrict__ box, int* __restrict__ size,
-    int* __restrict__ local, double* __restrict__ crit, bool collapse)
+    int* __restrict__ local, double* __restrict__ crit, bool collapse,
+    const int* __restrict__ order, const int* __restrict__ start)
 {
     using Key = typename Layout<BITS>::Key;
     const double rcx = box[0], rcy = box[1], rcz = box[2], side = box[3];
     const Key* prefix = static_cast<const Key*>(o.prefix);
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
-    for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-         v < o.n_nodes; v += stride) {
-        if (o.level[v] != L)
-            continue;
+    const int end = start[L + 1];
+    for (int k = start[L] + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+         k < end; k += stride) {
+        const int v = order[k];
         double gx, gy, gz, gh;
         int code = 0;
         node_geometry<BITS>(prefix[v], L, rcx, rcy, rcz, side, gx, gy, gz, gh,
@@ -383,13 +384,14 @@ __global__ __launch_bounds__(256) void level_down_kernel(
     const double* __restrict__ crit, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
     const double* __restrict__ box, double G, bool need_center, bool soft,
-    WalkOut w, int* __restrict__ wcount)
+    WalkOut w, int* __restrict__ wcount, const int* __restrict__ order,
+    const int* __restrict__ start)
 {
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
-    for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-         v < o.n_nodes; v += stride) {
-        if (o.level[v] != L)
-            continue;
+    const int end = start[L + 1];
+    for (int k = start[L] + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+         k < end; k += stride) {
+        const int v = order[k];
         int pre = 0;
         if (L == 0)
             *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
@@ -400,6 +402,105 @@ __global__ __launch_bounds__(256) void level_down_kernel(
             continue;
         write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32,
                                    box, G, need_center, soft, w);
+    }
+}
+
+// ------------------------------------------------- nodes grouped by level
+// The octree numbers nodes by radix-tree node, not by level.  A counting
+// sort groups them by level once per build, so that each level pass reads
+// only its own nodes.  Order within a level is irrelevant: every node of a
+// level is processed independently.  Three launches with the same grid:
+// per-block level counts, one scan over (level, block), and a scatter
+// through per-block cursors in LDS (no global atomics).
+// info layout: [0, kMaxLevels + 1) level starts (start[L + 1] ends level L),
+// then the kMaxLevels x blocks per-block counts / bases, level-major.
+constexpr int kMaxLevels = 32; // 63-bit keys give at most 22 levels
+constexpr int kLevelBlocks = 256;
+constexpr int kLevelCounts = kMaxLevels * kLevelBlocks;
+constexpr int kLevelInfoInts = kMaxLevels + 1 + kLevelCounts;
+
+__device__ __forceinline__ int node_level(const nbh_octree_view& o, int v)
+{
+    const int L = o.level[v];
+    return (L >= 0 && L < kMaxLevels) ? L : -1;
+}
+
+__global__ __launch_bounds__(256) void level_count_kernel(nbh_octree_view o,
+                                                          int* __restrict__ info)
+{
+    __shared__ int cnt[kMaxLevels];
+    if (threadIdx.x < kMaxLevels)
+        cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+         v < o.n_nodes; v += stride) {
+        const int L = node_level(o, v);
+        if (L >= 0)
+            atomicAdd(&cnt[L], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kMaxLevels)
+        info[kMaxLevels + 1 + threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// One block of 1024 threads: exclusive scan of the level-major counts (zero
+// beyond `blocks`) staged in LDS.
+__global__ __launch_bounds__(1024) void level_scan_kernel(int* __restrict__ info,
+                                                          int blocks)
+{
+    constexpr int kPer = kLevelCounts / 1024;
+    __shared__ int buf[kLevelCounts];
+    __shared__ int part[1024];
+    int* bc = info + kMaxLevels + 1;
+    const int total = kMaxLevels * blocks;
+    const int t = static_cast<int>(threadIdx.x);
+    for (int k = t; k < kLevelCounts; k += 1024)
+        buf[k] = k < total ? bc[k] : 0;
+    __syncthreads();
+    int sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+        sum += buf[t * kPer + j];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int add = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int c = buf[t * kPer + j];
+        buf[t * kPer + j] = run;
+        run += c;
+    }
+    __syncthreads();
+    for (int k = t; k < total; k += 1024)
+        bc[k] = buf[k];
+    if (t < kMaxLevels)
+        info[t] = buf[t * blocks];
+    if (t == 0)
+        info[kMaxLevels] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void level_scatter_kernel(nbh_octree_view o,
+                                                            const int* __restrict__ info,
+                                                            int* __restrict__ order)
+{
+    __shared__ int cursor[kMaxLevels];
+    if (threadIdx.x < kMaxLevels)
+        cursor[threadIdx.x] =
+            info[kMaxLevels + 1 + threadIdx.x * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+         v < o.n_nodes; v += stride) {
+        const int L = node_level(o, v);
+        if (L >= 0)
+            order[atomicAdd(&cursor[L], 1)] = v;
     }
 }
 
@@ -463,14 +564,15 @@ void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
 }
 
 // The pipeline's mass/COM + walk layout: levels max_level..0 bottom-up, then
-// 0..max_level top-down.  Each launch strides over all nodes and handles
-// those of its level.
+// 0..max_level top-down.  Each launch strides over the node list of its
+// level (`order`, grouped by a counting sort over the node levels).
 template <int BITS>
 void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          bool mixed, const double4* rec64, const int4* rec32,
                          double theta, const double* box, double G,
                          bool need_center, bool soft, int* size, int* local, double* crit,
-                         int* pre, double4* w64node, double* w64crit,
+                         int* pre, int* order, int* level_info,
+                         double4* w64node, double* w64crit,
                          int4* w64link, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
                          hipEvent_t between)
@@ -484,12 +586,25 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     // 2/sqrt(3) accepting the parent implies accepting the child, so every
     // lane's interaction list is unchanged.
     const bool collapse = mixed && theta < kContainsTheta;
+    if (max_level >= kMaxLevels)
+        fail(NBH_LOGIC_ERROR, "octree depth exceeds the level-pass limit");
     unsigned blocks = blocks_for(o.n_nodes, 256);
     if (blocks > 2048)
         blocks = 2048;
+    unsigned lblocks = blocks_for(o.n_nodes, 256);
+    if (lblocks > kLevelBlocks)
+        lblocks = kLevelBlocks;
+    level_count_kernel<<<lblocks, 256, 0, s>>>(o, level_info);
+    NBH_HIP(hipGetLastError());
+    level_scan_kernel<<<1, 1024, 0, s>>>(level_info, static_cast<int>(lblocks));
+    NBH_HIP(hipGetLastError());
+    level_scatter_kernel<<<lblocks, 256, 0, s>>>(o, level_info, order);
+    NBH_HIP(hipGetLastError());
+    const int* start = level_info;
     for (int L = max_level; L >= 0; --L) {
         level_up_kernel<BITS><<<blocks, 256, 0, s>>>(o, L, rec64, theta, box, size,
-                                                     local, crit, collapse);
+                                                     local, crit, collapse, order,
+                                                     start);
         NBH_HIP(hipGetLastError());
     }
     if (between)
@@ -508,20 +623,23 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
         if (mixed)
             level_down_kernel<true><<<blocks, 256, 0, s>>>(
                 o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
-                need_center, soft, w, wcount);
+                need_center, soft, w, wcount, order, start);
         else
             level_down_kernel<false><<<blocks, 256, 0, s>>>(
                 o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
-                need_center, soft, w, wcount);
+                need_center, soft, w, wcount, order, start);
         NBH_HIP(hipGetLastError());
     }
 }
+
+size_t level_info_ints() { return kLevelInfoInts; }
 
 #define NBH_LEVEL_INST(B)                                                            \
     template void launch_level_passes<B>(hipStream_t, const nbh_octree_view&, int, bool, \
                                          const double4*, const int4*, double,           \
                                          const double*, double, bool, bool, int*, int*, \
-                                         double*, int*, double4*, double*, int4*,       \
+                                         double*, int*, int*, int*, double4*, double*,  \
+                                         int4*,                                          \
                                          double4*, int4*, int4*, int4*, int*, hipEvent_t);
 NBH_LEVEL_INST(10)
 NBH_LEVEL_INST(21)

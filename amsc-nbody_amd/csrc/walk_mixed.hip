@@ -26,7 +26,7 @@ namespace nbh {
 
 namespace {
 
-constexpr int kFold = 16;
+constexpr int kFold = 32;
 
 // One position of the mixed walk: the node (fixed-point COM, G m) and its
 // link (crit^2, skip, kind | count << 8, first particle).
