@@ -118,6 +118,7 @@ void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scr
     scratch.ensure(force_vjp_scratch_doubles(n, M, M));
     const bool contains = s->cfg.theta >= kContainsTheta;
     const double eps = s->cfg.softening;
+    sim_fill_views(s); // the push-down reads node masses
     nbh_octree_view o = s->octree_view();
     launch_force_vjp_fp64(s->stream, o, s->cfg.bits_per_axis, s->pre.p,
                           s->w64node.p, s->w64crit.p, s->w64link.p,

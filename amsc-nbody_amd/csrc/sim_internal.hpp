@@ -15,12 +15,15 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          double theta, const double* box, double G,
                          bool need_center, bool soft, int* size, int* local,
                          double* crit, int* pre, int* order, int* level_info,
-                         double4* w64node, double* w64crit,
+                         double4* sum4, double4* w64node, double* w64crit,
                          int4* w64link, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
                          hipEvent_t between);
 // ints of the level-pass bookkeeping array (level starts + per-block counts)
 size_t level_info_ints();
+template <int BITS>
+void launch_octree_views(hipStream_t s, const nbh_octree_view& o,
+                         const double4* sum4, const double* box);
 
 template <typename T>
 struct DevBuf {
@@ -131,6 +134,8 @@ struct nbh_sim {
     nbh::DevBuf<unsigned char> prefix;
     nbh::DevBuf<double> mass, comx, comy, comz, cenx, ceny, cenz, half, crit;
     nbh::DevBuf<double4> w64node, w64center;
+    nbh::DevBuf<double4> sum4; // packed {com, mass} per node (level passes)
+    bool views_fresh = false;  // reference-layout node arrays match sum4
     nbh::DevBuf<double> w64crit;
     nbh::DevBuf<int4> w64link, w32node, w32center;
 
@@ -196,6 +201,9 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters);
 // Ends a step: reads the device error words (synchronises) and the phase
 // timings.
 void finish(nbh_sim* s);
+// Fills the reference-layout octree arrays (mass, centre of mass, cell
+// geometry, child counts) of the last rebuild, once per rebuild.
+void sim_fill_views(nbh_sim* s);
 // Allocates a simulation of `count` particles with zero accelerations and
 // unset particle state (validates the configuration and the count).
 nbh_sim* sim_new(const nbh_sim_config* config, size_t count, void* hip_stream);

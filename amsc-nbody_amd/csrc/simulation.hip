@@ -100,6 +100,7 @@ void ensure_octree(nbh_sim* s, size_t m)
                     &s->child_count, &s->size, &s->local, &s->pre, &s->order})
         b->alloc(cap);
     s->level_info.alloc(level_info_ints());
+    s->sum4.alloc(cap);
     s->prefix.alloc(cap * s->key_bytes);
     for (auto* b : {&s->mass, &s->comx, &s->comy, &s->comz, &s->cenx, &s->ceny,
                     &s->cenz, &s->half, &s->crit})
@@ -128,6 +129,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
     double* const* a = nullptr;
     (void)a;
     s->mark(PH_KICK_DRIFT_BBOX);
+    s->views_fresh = false;
     launch_fill_u32(st, s->zero.p, 0u, s->zero_words);
     unsigned* counter = reinterpret_cast<unsigned*>(s->scalars.p);
     if (kick_drift) {
@@ -204,7 +206,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
     launch_level_passes<BITS>(
         st, o, Layout<BITS>::bits, mixed(s), s->rec64.p, s->rec32.p,
         s->cfg.theta, s->box.p, s->cfg.gravitational_constant, contains,
-        s->cfg.softening > 0.0, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->w64node.p, s->w64crit.p,
+        s->cfg.softening > 0.0, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->sum4.p, s->w64node.p, s->w64crit.p,
         s->w64link.p, s->w64center.p, s->w32node.p, nullptr,
         s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr);
 }
@@ -257,6 +259,18 @@ void finish(nbh_sim* s)
                 s->phase_ms[p] = 0.0;
         }
     }
+}
+
+void sim_fill_views(nbh_sim* s)
+{
+    if (s->views_fresh || s->nnodes <= 0)
+        return;
+    nbh_octree_view o = s->octree_view();
+    if (s->cfg.bits_per_axis == 10)
+        launch_octree_views<10>(s->stream, o, s->sum4.p, s->box.p);
+    else
+        launch_octree_views<21>(s->stream, o, s->sum4.p, s->box.p);
+    s->views_fresh = true;
 }
 
 void do_rebuild(nbh_sim* s, bool kick_drift)
@@ -480,6 +494,8 @@ int nbh_sim_structure_view(nbh_sim* s, nbh_sim_structure* out)
             fail(NBH_INVALID_ARGUMENT, "arguments must not be null");
         if (s->state == 0)
             fail(NBH_LOGIC_ERROR, "Simulation has no structure before initialize");
+        sim_fill_views(s);
+        NBH_HIP(hipStreamSynchronize(s->stream));
         std::memset(out, 0, sizeof(*out));
         out->n_particles = static_cast<int>(s->n);
         out->n_groups = s->ngroups;

@@ -135,14 +135,15 @@ struct WalkOut {
 // The record of node v at pre-order position `pre`.  GENERIC: a
 // caller-supplied octree, with the reference's per-node failure conditions
 // encoded as error kinds and the opening radius computed here.  Otherwise:
-// the pipeline's own octree with its precomputed opening radius.
+// the pipeline's own octree with its precomputed opening radius, packed
+// {com, mass} in sum4[v] and the cell geometry {centre, half} in `geom`.
 template <bool GENERIC, bool MIXED>
 __device__ __forceinline__ void write_record(
     const nbh_octree_view& o, int v, int pre, const int* __restrict__ size,
     const double* __restrict__ crit_in, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
     const double* __restrict__ box, double G, bool need_center, bool soft,
-    const WalkOut& w)
+    const WalkOut& w, const double4* __restrict__ sum4, double4 geom)
 {
     double4* w64node = w.w64node;
     double* w64crit = w.w64crit;
@@ -153,7 +154,12 @@ __device__ __forceinline__ void write_record(
     int4* w32center = w.w32center;
     const int skip = pre + size[v];
     const int pc = o.particle_count[v];
-    const double h = o.half_size[v];
+    const double h = GENERIC ? o.half_size[v] : geom.w;
+    double4 mc;
+    if (GENERIC)
+        mc = make_double4(o.com_x[v], o.com_y[v], o.com_z[v], o.mass[v]);
+    else
+        mc = sum4[v];
     int kind;
     int first = 0, count = 0;
     if (GENERIC) {
@@ -182,14 +188,14 @@ __device__ __forceinline__ void write_record(
             kind = KIND_LEAF;
             first = o.first_particle[v];
             count = pc;
-        } else if (o.mass[v] == 0.0) {
+        } else if (mc.w == 0.0) {
             kind = KIND_SKIP;
         } else {
             kind = KIND_INTERNAL;
         }
     }
-    const double m = o.mass[v];
-    const double cx = o.com_x[v], cy = o.com_y[v], cz = o.com_z[v];
+    const double m = mc.w;
+    const double cx = mc.x, cy = mc.y, cz = mc.z;
     double crit = 0.0;
     if (kind == KIND_INTERNAL || kind == KIND_CHILDLESS) {
         if (GENERIC) {
@@ -210,8 +216,9 @@ __device__ __forceinline__ void write_record(
         w64crit[pre] = crit;
         w64link[pre] = make_int4(skip, kind, first, count);
         if (need_center)
-            w64center[pre] =
-                make_double4(o.center_x[v], o.center_y[v], o.center_z[v], h);
+            w64center[pre] = GENERIC ? make_double4(o.center_x[v], o.center_y[v],
+                                                    o.center_z[v], h)
+                                     : geom;
     } else {
         const double side = box[3];
         const double scale = 2147483648.0 / side;
@@ -244,9 +251,12 @@ __device__ __forceinline__ void write_record(
         w32link[2 * pre] = make_int4(__float_as_int(crit2), skip,
                                  kind_out | (count << 8), first);
         if (need_center) {
-            const int qx = static_cast<int>(rint((o.center_x[v] - ox) * scale));
-            const int qy = static_cast<int>(rint((o.center_y[v] - oy) * scale));
-            const int qz = static_cast<int>(rint((o.center_z[v] - oz) * scale));
+            const double gx = GENERIC ? o.center_x[v] : geom.x;
+            const double gy = GENERIC ? o.center_y[v] : geom.y;
+            const double gz = GENERIC ? o.center_z[v] : geom.z;
+            const int qx = static_cast<int>(rint((gx - ox) * scale));
+            const int qy = static_cast<int>(rint((gy - oy) * scale));
+            const int qz = static_cast<int>(rint((gz - oz) * scale));
             const int qh = fast ? -1 : static_cast<int>(fmin(rint(h * scale), lim));
             w32center[pre] = make_int4(qx, qy, qz, qh);
         }
@@ -270,7 +280,8 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
     if (v == 0)
         *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
     write_record<GENERIC, MIXED>(o, v, pre, size, crit_in, theta, rec64, rec32,
-                                 box, G, need_center, false, w);
+                                 box, G, need_center, false, w, nullptr,
+                                 make_double4(0.0, 0.0, 0.0, 0.0));
 }
 
 // ------------------------------------------ level-synchronous passes
@@ -288,7 +299,8 @@ __global__ __launch_bounds__(256) void level_up_kernel(
     nbh_octree_view o, int L, const double4* __restrict__ rec, double theta,
     const double* __restrict__ box, int* __restrict__ size,
     int* __restrict__ local, double* __restrict__ crit, bool collapse,
-    const int* __restrict__ order, const int* __restrict__ start)
+    const int* __restrict__ order, const int* __restrict__ start,
+    double4* __restrict__ sum4)
 {
     using Key = typename Layout<BITS>::Key;
     const double rcx = box[0], rcy = box[1], rcz = box[2], side = box[3];
@@ -302,10 +314,6 @@ __global__ __launch_bounds__(256) void level_up_kernel(
         int code = 0;
         node_geometry<BITS>(prefix[v], L, rcx, rcy, rcz, side, gx, gy, gz, gh,
                             code);
-        o.center_x[v] = gx;
-        o.center_y[v] = gy;
-        o.center_z[v] = gz;
-        o.half_size[v] = gh;
         double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
         const int pc = o.particle_count[v];
         int nch = 0, only = 0;
@@ -327,11 +335,11 @@ __global__ __launch_bounds__(256) void level_up_kernel(
             for (int oc = 0; oc < 8; ++oc) {
                 if (ch[oc] == -1)
                     continue;
-                const double cm = o.mass[ch[oc]];
-                tm += cm;
-                wx = fma(cm, o.com_x[ch[oc]], wx);
-                wy = fma(cm, o.com_y[ch[oc]], wy);
-                wz = fma(cm, o.com_z[ch[oc]], wz);
+                const double4 c = sum4[ch[oc]];
+                tm += c.w;
+                wx = fma(c.w, c.x, wx);
+                wy = fma(c.w, c.y, wy);
+                wz = fma(c.w, c.z, wz);
                 only = ch[oc];
                 ++nch;
             }
@@ -357,12 +365,7 @@ __global__ __launch_bounds__(256) void level_up_kernel(
             cy = wy * inv;
             cz = wz * inv;
         }
-        o.mass[v] = tm;
-        o.com_x[v] = cx;
-        o.com_y[v] = cy;
-        o.com_z[v] = cz;
-        o.child_count[v] = nch;
-        o.pending_children[v] = nch;
+        sum4[v] = make_double4(cx, cy, cz, tm);
         if (pc <= 0) {
             const double ox = cx - gx;
             const double oy = cy - gy;
@@ -377,7 +380,7 @@ __global__ __launch_bounds__(256) void level_up_kernel(
 
 // Top-down over one level: pre-order position = parent's + own offset, then
 // the node's walk record.
-template <bool MIXED>
+template <int BITS, bool MIXED>
 __global__ __launch_bounds__(256) void level_down_kernel(
     nbh_octree_view o, int L, const int* __restrict__ size,
     const int* __restrict__ local, int* __restrict__ pre_of,
@@ -385,8 +388,10 @@ __global__ __launch_bounds__(256) void level_down_kernel(
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
     const double* __restrict__ box, double G, bool need_center, bool soft,
     WalkOut w, int* __restrict__ wcount, const int* __restrict__ order,
-    const int* __restrict__ start)
+    const int* __restrict__ start, const double4* __restrict__ sum4)
 {
+    using Key = typename Layout<BITS>::Key;
+    const Key* prefix = static_cast<const Key*>(o.prefix);
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
     const int end = start[L + 1];
     for (int k = start[L] + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -400,8 +405,14 @@ __global__ __launch_bounds__(256) void level_down_kernel(
         pre_of[v] = pre;
         if (pre < 0 || pre >= o.n_nodes)
             continue;
+        double4 geom = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (need_center) {
+            int code = 0;
+            node_geometry<BITS>(prefix[v], L, box[0], box[1], box[2], box[3],
+                                geom.x, geom.y, geom.z, geom.w, code);
+        }
         write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32,
-                                   box, G, need_center, soft, w);
+                                   box, G, need_center, soft, w, sum4, geom);
     }
 }
 
@@ -504,6 +515,37 @@ __global__ __launch_bounds__(256) void level_scatter_kernel(nbh_octree_view o,
     }
 }
 
+template <int BITS>
+__global__ __launch_bounds__(256) void octree_views_kernel(
+    nbh_octree_view o, const double4* __restrict__ sum4,
+    const double* __restrict__ box)
+{
+    using Key = typename Layout<BITS>::Key;
+    const int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= o.n_nodes)
+        return;
+    const Key* prefix = static_cast<const Key*>(o.prefix);
+    double gx = 0.0, gy = 0.0, gz = 0.0, gh = 0.0;
+    int code = 0;
+    node_geometry<BITS>(prefix[v], o.level[v], box[0], box[1], box[2], box[3],
+                        gx, gy, gz, gh, code);
+    o.center_x[v] = gx;
+    o.center_y[v] = gy;
+    o.center_z[v] = gz;
+    o.half_size[v] = gh;
+    const double4 mc = sum4[v];
+    o.mass[v] = mc.w;
+    o.com_x[v] = mc.x;
+    o.com_y[v] = mc.y;
+    o.com_z[v] = mc.z;
+    int nch = 0;
+    if (o.particle_count[v] <= 0)
+        for (int oc = 0; oc < 8; ++oc)
+            nch += o.children[8 * v + oc] != -1;
+    o.child_count[v] = nch;
+    o.pending_children[v] = nch;
+}
+
 // Every link starts as "error 4, skip to the end", so a position that the
 // build leaves unwritten (a node off the root's tree) stops the walk with a
 // topology error instead of reading an undefined record.
@@ -536,6 +578,8 @@ void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
                        int4* w64link, double4* w64center, int4* w32node,
                        int4* w32link, int4* w32center, int* wcount)
 {
+    if (!generic)
+        fail(NBH_LOGIC_ERROR, "the per-node walk build serves caller octrees only");
     const unsigned blocks = blocks_for(o.n_nodes, 256);
     const int M = o.n_nodes;
     // Mixed records are interleaved {node, link} pairs (see below).
@@ -551,14 +595,10 @@ void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
     const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32pair_link,
                     w32center};
 #define NBH_WALK_ARGS o, size, local, crit, theta, rec64, rec32, box, G, need_center, w, wcount
-    if (generic && !mixed)
+    if (!mixed)
         walk_build_kernel<true, false><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
-    else if (generic && mixed)
-        walk_build_kernel<true, true><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
-    else if (!mixed)
-        walk_build_kernel<false, false><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
     else
-        walk_build_kernel<false, true><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
+        walk_build_kernel<true, true><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
 #undef NBH_WALK_ARGS
     NBH_HIP(hipGetLastError());
 }
@@ -571,7 +611,7 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          bool mixed, const double4* rec64, const int4* rec32,
                          double theta, const double* box, double G,
                          bool need_center, bool soft, int* size, int* local, double* crit,
-                         int* pre, int* order, int* level_info,
+                         int* pre, int* order, int* level_info, double4* sum4,
                          double4* w64node, double* w64crit,
                          int4* w64link, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
@@ -604,7 +644,7 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     for (int L = max_level; L >= 0; --L) {
         level_up_kernel<BITS><<<blocks, 256, 0, s>>>(o, L, rec64, theta, box, size,
                                                      local, crit, collapse, order,
-                                                     start);
+                                                     start, sum4);
         NBH_HIP(hipGetLastError());
     }
     if (between)
@@ -621,25 +661,40 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                     w32center};
     for (int L = 0; L <= max_level; ++L) {
         if (mixed)
-            level_down_kernel<true><<<blocks, 256, 0, s>>>(
+            level_down_kernel<BITS, true><<<blocks, 256, 0, s>>>(
                 o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
-                need_center, soft, w, wcount, order, start);
+                need_center, soft, w, wcount, order, start, sum4);
         else
-            level_down_kernel<false><<<blocks, 256, 0, s>>>(
+            level_down_kernel<BITS, false><<<blocks, 256, 0, s>>>(
                 o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
-                need_center, soft, w, wcount, order, start);
+                need_center, soft, w, wcount, order, start, sum4);
         NBH_HIP(hipGetLastError());
     }
 }
 
 size_t level_info_ints() { return kLevelInfoInts; }
 
+// The reference-layout per-node arrays of the pipeline's octree (mass and
+// centre of mass, cell centre and half size, child counts), filled from the
+// packed sums on request: the level passes themselves only keep sum4.
+template <int BITS>
+void launch_octree_views(hipStream_t s, const nbh_octree_view& o,
+                         const double4* sum4, const double* box)
+{
+    octree_views_kernel<BITS><<<blocks_for(o.n_nodes, 256), 256, 0, s>>>(o, sum4, box);
+    NBH_HIP(hipGetLastError());
+}
+template void launch_octree_views<10>(hipStream_t, const nbh_octree_view&,
+                                      const double4*, const double*);
+template void launch_octree_views<21>(hipStream_t, const nbh_octree_view&,
+                                      const double4*, const double*);
+
 #define NBH_LEVEL_INST(B)                                                            \
     template void launch_level_passes<B>(hipStream_t, const nbh_octree_view&, int, bool, \
                                          const double4*, const int4*, double,           \
                                          const double*, double, bool, bool, int*, int*, \
-                                         double*, int*, int*, int*, double4*, double*,  \
-                                         int4*,                                          \
+                                         double*, int*, int*, int*, double4*, double4*, \
+                                         double*, int4*,                                 \
                                          double4*, int4*, int4*, int4*, int*, hipEvent_t);
 NBH_LEVEL_INST(10)
 NBH_LEVEL_INST(21)

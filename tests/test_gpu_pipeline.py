@@ -214,3 +214,27 @@ def test_binding_imported_before_torch_uses_one_runtime():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     assert r.stdout.startswith("ok")
+
+
+def test_fp64_pipeline_bit_exact_uniform_1m(nb, orc):
+    # The 1M-particle uniform workload at full size: keys, permutation,
+    # octree, mass/COM, accelerations and the state after two steps equal the
+    # oracle's bit for bit.
+    n = 1 << 20
+    p = make(nb, "uniform", n, seed=1)
+    cfg = dict(time_step=1e-4, theta=0.5, softening=1e-3)
+    sim = nb.Simulation(p, gravitational_constant=1.0, precision=nb.PRECISION_FP64,
+                        bits_per_axis=21, **cfg)
+    ref = orc.Sim(p, cfg["time_step"], cfg["theta"], cfg["softening"], 1.0, 21)
+    sim.initialize()
+    ref.initialize()
+    compare_structure(sim.structure_host(), ref.structure())
+    sim.run(2)
+    ref.step(2)
+    compare_structure(sim.structure_host(), ref.structure())
+    rs, st = ref.state(), sim.state()
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(st[f], rs[f]), f
+    a = sim.accelerations()
+    for k, f in enumerate(("ax", "ay", "az")):
+        assert np.array_equal(a[k], rs[f]), f
