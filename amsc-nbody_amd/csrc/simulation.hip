@@ -411,6 +411,36 @@ int nbh_sim_create(nbh_sim** out, const nbh_sim_config* config, size_t count,
     });
 }
 
+int nbh_sim_create_device(nbh_sim** out, const nbh_sim_config* config,
+                          size_t count, const double* d_mass, const double* d_x,
+                          const double* d_y, const double* d_z,
+                          const double* d_vx, const double* d_vy,
+                          const double* d_vz, void* hip_stream)
+{
+    return guard([&] {
+        if (!out)
+            fail(NBH_INVALID_ARGUMENT, "output pointer must not be null");
+        validate_config(config);
+        if (count != 0 && (!d_mass || !d_x || !d_y || !d_z || !d_vx || !d_vy || !d_vz))
+            fail(NBH_INVALID_ARGUMENT, "particle arrays must not be null");
+        nbh_sim* s = sim_new(config, count, hip_stream);
+        try {
+            const size_t bytes = count * sizeof(double);
+            const double* src[7] = {d_mass, d_x, d_y, d_z, d_vx, d_vy, d_vz};
+            nbh::DevBuf<double>* dst[7] = {&s->m, &s->x, &s->y, &s->z,
+                                           &s->vx, &s->vy, &s->vz};
+            for (int k = 0; k < 7; ++k)
+                NBH_HIP(hipMemcpyAsync(dst[k]->p, src[k], bytes,
+                                       hipMemcpyDeviceToDevice, s->stream));
+            NBH_HIP(hipStreamSynchronize(s->stream));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
 int nbh_sim_destroy(nbh_sim* sim)
 {
     return guard([&] { delete sim; });

@@ -253,6 +253,12 @@ int nbh_sim_create(nbh_sim** out, const nbh_sim_config* config, size_t count,
                    const double* h_mass, const double* h_x, const double* h_y,
                    const double* h_z, const double* h_vx, const double* h_vy,
                    const double* h_vz, void* hip_stream);
+/* The same from device arrays (copied; the caller keeps ownership). */
+int nbh_sim_create_device(nbh_sim** out, const nbh_sim_config* config,
+                          size_t count, const double* d_mass, const double* d_x,
+                          const double* d_y, const double* d_z,
+                          const double* d_vx, const double* d_vy,
+                          const double* d_vz, void* hip_stream);
 int nbh_sim_destroy(nbh_sim* sim);
 /* Simulation::initialize / step / run (src/simulation.cpp:78-127). */
 int nbh_sim_initialize(nbh_sim* sim);
