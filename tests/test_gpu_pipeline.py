@@ -238,3 +238,25 @@ def test_fp64_pipeline_bit_exact_uniform_1m(nb, orc):
     a = sim.accelerations()
     for k, f in enumerate(("ax", "ay", "az")):
         assert np.array_equal(a[k], rs[f]), f
+
+
+def test_fp64_pipeline_bit_exact_plummer16k_ten_steps(nb, orc):
+    # The reference's CPU-runnable workload (Plummer N=16384, theta 0.5,
+    # eps 0.01, dt 1e-3, 10 steps) with 30-bit keys: state and accelerations
+    # after ten steps equal the oracle's bit for bit.
+    p = make(nb, "plummer", 16384, seed=42)
+    cfg = dict(time_step=1e-3, theta=0.5, softening=0.01)
+    sim = nb.Simulation(p, gravitational_constant=1.0, precision=nb.PRECISION_FP64,
+                        bits_per_axis=10, **cfg)
+    ref = orc.Sim(p, cfg["time_step"], cfg["theta"], cfg["softening"], 1.0, 10)
+    sim.initialize()
+    ref.initialize()
+    sim.run(10)
+    ref.step(10)
+    rs, st = ref.state(), sim.state()
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(st[f], rs[f]), f
+    a = sim.accelerations()
+    for k, f in enumerate(("ax", "ay", "az")):
+        assert np.array_equal(a[k], rs[f]), f
+    compare_structure(sim.structure_host(), ref.structure())
