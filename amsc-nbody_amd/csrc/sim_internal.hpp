@@ -15,7 +15,8 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          double theta, const double* box, double G,
                          bool need_center, bool soft, int* size, int* local,
                          double* crit, int* pre, int* order, int* level_info,
-                         double4* sum4, double4* w64node, double* w64crit,
+                         double4* sum4, const int* poffset, const int* pcount,
+                         int n_groups, double4* w64node, double* w64crit,
                          int4* w64link, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
                          hipEvent_t between);

@@ -206,7 +206,8 @@ void rebuild(nbh_sim* s, bool kick_drift)
     launch_level_passes<BITS>(
         st, o, Layout<BITS>::bits, mixed(s), s->rec64.p, s->rec32.p,
         s->cfg.theta, s->box.p, s->cfg.gravitational_constant, contains,
-        s->cfg.softening > 0.0, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->sum4.p, s->w64node.p, s->w64crit.p,
+        s->cfg.softening > 0.0, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->sum4.p, s->poffset.p, s->pcount.p, s->ngroups,
+        s->w64node.p, s->w64crit.p,
         s->w64link.p, s->w64center.p, s->w32node.p, nullptr,
         s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr);
 }

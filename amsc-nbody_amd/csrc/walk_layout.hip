@@ -430,8 +430,11 @@ constexpr int kLevelBlocks = 256;
 constexpr int kLevelCounts = kMaxLevels * kLevelBlocks;
 constexpr int kLevelInfoInts = kMaxLevels + 1 + kLevelCounts;
 
+// Internal nodes only: leaves are handled by the per-group leaf passes.
 __device__ __forceinline__ int node_level(const nbh_octree_view& o, int v)
 {
+    if (o.particle_count[v] > 0)
+        return -1;
     const int L = o.level[v];
     return (L >= 0 && L < kMaxLevels) ? L : -1;
 }
@@ -546,6 +549,91 @@ __global__ __launch_bounds__(256) void octree_views_kernel(
     o.pending_children[v] = nch;
 }
 
+// ------------------------------------------------------------ leaf passes
+// Leaves depend on their own particles only, so they need no level order:
+// one thread per Morton group g, whose leaf is the last chain node of radix
+// leaf g, node edgeNodeOffset[g] + edgeNodeCount[g] (octree numbering of
+// octree_builder.cu:49-224).  Groups, their particles and their leaves are
+// all in Morton order, so these passes read and write nearly sequentially.
+__device__ __forceinline__ int leaf_of_group(const int* __restrict__ poffset,
+                                             const int* __restrict__ pcount,
+                                             const nbh_octree_view& o, int g)
+{
+    const int v = poffset[g] + pcount[g];
+    return (v > 0 && v < o.n_nodes && o.particle_count[v] > 0) ? v : -1;
+}
+
+// Leaf sums (octree_physics.cu:94-153, particles in sorted order) and
+// subtree size 1, ahead of the internal levels.
+__global__ __launch_bounds__(256) void leaf_up_kernel(
+    nbh_octree_view o, const int* __restrict__ poffset,
+    const int* __restrict__ pcount, int n_groups,
+    const double4* __restrict__ rec, double4* __restrict__ sum4,
+    int* __restrict__ size)
+{
+    const int g = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= n_groups)
+        return;
+    const int v = leaf_of_group(poffset, pcount, o, g);
+    if (v < 0)
+        return;
+    const int f = o.first_particle[v];
+    const int pc = o.particle_count[v];
+    double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+    for (int k = 0; k < pc; ++k) {
+        const double4 p = rec[f + k];
+        tm += p.w;
+        wx = fma(p.w, p.x, wx);
+        wy = fma(p.w, p.y, wy);
+        wz = fma(p.w, p.z, wz);
+    }
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    if (tm > 0.0) {
+        const double inv = 1.0 / tm;
+        cx = wx * inv;
+        cy = wy * inv;
+        cz = wz * inv;
+    }
+    sum4[v] = make_double4(cx, cy, cz, tm);
+    size[v] = 1;
+}
+
+// Leaf positions and records, after every internal level has its position.
+template <int BITS, bool MIXED>
+__global__ __launch_bounds__(256) void leaf_down_kernel(
+    nbh_octree_view o, const int* __restrict__ poffset,
+    const int* __restrict__ pcount, int n_groups, const int* __restrict__ size,
+    const int* __restrict__ local, int* __restrict__ pre_of,
+    const double* __restrict__ crit, double theta,
+    const double4* __restrict__ rec64, const int4* __restrict__ rec32,
+    const double* __restrict__ box, double G, bool need_center, bool soft,
+    WalkOut w, const double4* __restrict__ sum4)
+{
+    using Key = typename Layout<BITS>::Key;
+    const int g = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= n_groups)
+        return;
+    const int v = leaf_of_group(poffset, pcount, o, g);
+    if (v < 0)
+        return;
+    const int parent = o.parent[v];
+    if (parent < 0 || parent >= o.n_nodes)
+        return;
+    const int pre = pre_of[parent] + local[v];
+    pre_of[v] = pre;
+    if (pre < 0 || pre >= o.n_nodes)
+        return;
+    double4 geom = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (need_center) {
+        int code = 0;
+        const Key* prefix = static_cast<const Key*>(o.prefix);
+        node_geometry<BITS>(prefix[v], o.level[v], box[0], box[1], box[2], box[3],
+                            geom.x, geom.y, geom.z, geom.w, code);
+    }
+    write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32, box, G,
+                               need_center, soft, w, sum4, geom);
+}
+
 // Every link starts as "error 4, skip to the end", so a position that the
 // build leaves unwritten (a node off the root's tree) stops the walk with a
 // topology error instead of reading an undefined record.
@@ -612,6 +700,7 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          double theta, const double* box, double G,
                          bool need_center, bool soft, int* size, int* local, double* crit,
                          int* pre, int* order, int* level_info, double4* sum4,
+                         const int* poffset, const int* pcount, int n_groups,
                          double4* w64node, double* w64crit,
                          int4* w64link, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
@@ -641,6 +730,10 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     level_scatter_kernel<<<lblocks, 256, 0, s>>>(o, level_info, order);
     NBH_HIP(hipGetLastError());
     const int* start = level_info;
+    const unsigned gblocks = blocks_for(n_groups > 0 ? n_groups : 1, 256);
+    leaf_up_kernel<<<gblocks, 256, 0, s>>>(o, poffset, pcount, n_groups, rec64, sum4,
+                                           size);
+    NBH_HIP(hipGetLastError());
     for (int L = max_level; L >= 0; --L) {
         level_up_kernel<BITS><<<blocks, 256, 0, s>>>(o, L, rec64, theta, box, size,
                                                      local, crit, collapse, order,
@@ -670,6 +763,17 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                 need_center, soft, w, wcount, order, start, sum4);
         NBH_HIP(hipGetLastError());
     }
+    // Leaves last: a leaf that collapsed its single-child parent overwrites
+    // the parent's record at the shared position.
+    if (mixed)
+        leaf_down_kernel<BITS, true><<<gblocks, 256, 0, s>>>(
+            o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
+            box, G, need_center, soft, w, sum4);
+    else
+        leaf_down_kernel<BITS, false><<<gblocks, 256, 0, s>>>(
+            o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
+            box, G, need_center, soft, w, sum4);
+    NBH_HIP(hipGetLastError());
 }
 
 size_t level_info_ints() { return kLevelInfoInts; }
@@ -693,7 +797,8 @@ template void launch_octree_views<21>(hipStream_t, const nbh_octree_view&,
     template void launch_level_passes<B>(hipStream_t, const nbh_octree_view&, int, bool, \
                                          const double4*, const int4*, double,           \
                                          const double*, double, bool, bool, int*, int*, \
-                                         double*, int*, int*, int*, double4*, double4*, \
+                                         double*, int*, int*, int*, double4*,           \
+                                         const int*, const int*, int, double4*,          \
                                          double*, int4*,                                 \
                                          double4*, int4*, int4*, int4*, int*, hipEvent_t);
 NBH_LEVEL_INST(10)
