@@ -182,6 +182,9 @@ _sig("nbh_generate", _i, _i, _sz, _u64, _p, _p, _p, _p, _p, _p, _p)
 _sig("nbh_copy_to_host", _i, _p, _p, _sz)
 _sig("nbh_direct_sum", _i, _p, _p, _p, _p, _p, _sz, _p, _sz, _d, _d, _p, _p, _p)
 _sig("nbh_sim_energy", _i, _p, C.POINTER(_d), C.POINTER(_d))
+_sig("nbh_particle_file_info", _i, C.c_char_p, C.POINTER(_sz), C.POINTER(_i))
+_sig("nbh_particle_file_read", _i, C.c_char_p, _sz, _p, _p, _p, _p, _p, _p, _p, _p,
+     C.POINTER(_sz))
 _sig("nbh_sim_potential", _i, _p, _p)
 _sig("nbh_sim_force_vjp", _i, _p, _p, _p, _p, _p, _p, _p)
 _sig("nbh_adjoint_create", _i, C.POINTER(_p), C.POINTER(SimConfig), _sz, _i, _i, _p)
@@ -663,6 +666,35 @@ def direct_sum(m, x, y, z, targets, softening, G=1.0, ctx=None):
     check(lib.nbh_direct_sum(ctx.handle, ptr(m), ptr(x), ptr(y), ptr(z), m.numel(),
                              ptr(targets), nt, softening, G, *[ptr(o) for o in out]))
     return out
+
+
+# ----------------------------------------------------------- particle files
+def load_particles(path) -> dict:
+    """Reads a planar binary particle file (the reference's input format) into
+    host arrays m, x, y, z, vx, vy, vz (float64) and type (uint8)."""
+    enc = os.fsencode(str(path))
+    n, has_types = _sz(0), _i(0)
+    check(lib.nbh_particle_file_info(enc, C.byref(n), C.byref(has_types)))
+    count = n.value
+    out = {k: np.empty(count, dtype=np.float64) for k in ("m", "x", "y", "z", "vx", "vy", "vz")}
+    out["type"] = np.empty(count, dtype=np.uint8)
+    got = _sz(0)
+    check(lib.nbh_particle_file_read(enc, count,
+                                     *[out[k].ctypes.data for k in ("m", "x", "y", "z", "vx",
+                                                                    "vy", "vz", "type")],
+                                     C.byref(got)))
+    return out
+
+
+def save_particles(path, particles: dict, wide_header=True, with_types=False):
+    """Writes host arrays in the planar binary particle format."""
+    n = len(particles["m"])
+    with open(path, "wb") as f:
+        f.write(np.array([n], dtype=np.uint64 if wide_header else np.uint32).tobytes())
+        for k in ("m", "x", "y", "z", "vx", "vy", "vz"):
+            f.write(np.ascontiguousarray(particles[k], dtype=np.float64).tobytes())
+        if with_types:
+            f.write(np.ascontiguousarray(particles["type"], dtype=np.uint8).tobytes())
 
 
 # --------------------------------------------------------------- workloads

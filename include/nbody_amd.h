@@ -359,6 +359,19 @@ int nbh_generate(int kind, size_t count, uint64_t seed, double* h_mass,
                  double* h_x, double* h_y, double* h_z, double* h_vx,
                  double* h_vy, double* h_vz);
 
+/* Planar binary particle files (src/particle.hpp:19-77,145-224): a uint64 or
+ * uint32 particle count, seven blocks of `count` doubles (mass, x, y, z, vx,
+ * vy, vz) and optionally `count` uint8 particle types; every value finite,
+ * masses non-negative, the file size exact.  info reports the count and
+ * whether a type block is present; read fills host arrays of at least
+ * `capacity` elements (h_type may be NULL) and sets *count.  Format and
+ * validation failures -> NBH_RUNTIME_ERROR with the reference's messages. */
+int nbh_particle_file_info(const char* path, size_t* count, int* has_types);
+int nbh_particle_file_read(const char* path, size_t capacity, double* h_mass,
+                           double* h_x, double* h_y, double* h_z, double* h_vx,
+                           double* h_vy, double* h_vz, uint8_t* h_type,
+                           size_t* count);
+
 /* FP64 direct sum on `n_targets` targets (indices into the particle arrays),
  * softened, on the device: the accuracy reference for large runs. */
 int nbh_direct_sum(nbh_context* ctx, const double* d_mass, const double* d_x,
