@@ -4,10 +4,14 @@
 // parse.  Exit status 0 on success, 2 for argument errors (with the usage on
 // stderr), 1 for any other failure.
 //
-// Graphical streaming (frame capture, the stream server) is not part of this
-// build: its options are parsed and validated, and enabling it is reported
-// as an argument error.  Additions: --precision fp64|mixed, --key-bits 30|63
-// and --gravitational-constant G (defaults: fp64, 30, the SI constant).
+// --stream-dir DIR writes sampled, quantized .nbsnap frames of the positions
+// at --sample-rate wall-clock Hz into a durable spool, as the reference does
+// (the initial state, then after steps when a frame is due, and the final
+// state), and marks the run complete.  The stream server (--cluster and its
+// network options) is not part of this build: those options are parsed and
+// validated, and --cluster is reported as an argument error.  Additions:
+// --precision fp64|mixed, --key-bits 30|63 and --gravitational-constant G
+// (defaults: fp64, 30, the SI constant).
 #include <cerrno>
 #include <charconv>
 #include <chrono>
@@ -17,6 +21,8 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <limits>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <string_view>
@@ -24,6 +30,8 @@
 
 #include "particle.hpp"
 #include "simulation.hpp"
+#include "streaming/device_capture.hpp"
+#include "streaming/frame.hpp"
 
 namespace {
 
@@ -34,8 +42,10 @@ struct Options {
     bool has_time_step = false;
     bool has_steps = false;
     bool help = false;
+    bool cluster = false;
     bool streaming = false;
     double sample_rate = 60.0;
+    std::size_t maximum_stream_particles = 100000;
     std::string stream_dir = "nbody-stream";
     std::string stream_bind = "127.0.0.1";
 };
@@ -54,10 +64,15 @@ void usage(std::ostream& out, const char* program)
         << "      --precision P      fp64 or mixed (FP32 traversal; default: fp64)\n"
         << "      --key-bits B       Morton key width, 30 or 63 (default: 30)\n"
         << "      --gravitational-constant G  (default: 6.6743e-11)\n\n"
-        << "Graphical stream options (parsed; streaming is not available in this build):\n"
-        << "      --cluster, --stream-dir DIR, --sample-rate HZ,\n"
-        << "      --stream-max-particles N, --stream-bind ADDR, --stream-port PORT,\n"
-        << "      --stream-server FILE\n\n"
+        << "Graphical stream options:\n"
+        << "      --stream-dir DIR   Durable frame queue; also enables output\n"
+        << "                         (default: nbody-stream)\n"
+        << "      --sample-rate HZ   Wall-clock frame rate (default: 60)\n"
+        << "      --stream-max-particles N  Display sample size; 0 sends all "
+           "(default: 100000)\n"
+        << "      --cluster, --stream-bind ADDR, --stream-port PORT, --stream-server FILE\n"
+        << "                         Stream server options (parsed; the server is not\n"
+        << "                         available in this build)\n\n"
         << "Other options:\n"
         << "  -h, --help             Show this help and exit\n";
 }
@@ -132,6 +147,7 @@ Options parse(int argc, char** argv)
         } else if (a == "--gravitational-constant") {
             o.config.gravitationalConstant = parse_double(value_of(argc, argv, i, a), a);
         } else if (a == "--cluster") {
+            o.cluster = true;
             o.streaming = true;
         } else if (a == "--stream-dir") {
             o.stream_dir = value_of(argc, argv, i, a);
@@ -139,8 +155,8 @@ Options parse(int argc, char** argv)
         } else if (a == "--sample-rate") {
             o.sample_rate = parse_double(value_of(argc, argv, i, a), a);
         } else if (a == "--stream-max-particles") {
-            (void)parse_unsigned<std::size_t>(value_of(argc, argv, i, a), a,
-                                              "non-negative integer", false);
+            o.maximum_stream_particles = parse_unsigned<std::size_t>(
+                value_of(argc, argv, i, a), a, "non-negative integer", false);
         } else if (a == "--stream-bind") {
             o.stream_bind = value_of(argc, argv, i, a);
         } else if (a == "--stream-port") {
@@ -175,10 +191,10 @@ Options parse(int argc, char** argv)
         throw std::invalid_argument("--stream-dir must not be empty");
     if (o.stream_bind.empty())
         throw std::invalid_argument("--stream-bind must not be empty");
-    if (o.streaming)
+    if (o.cluster)
         throw std::invalid_argument(
-            "graphical streaming (frame capture and the stream server) is not "
-            "available in this build");
+            "--cluster: the graphical stream server is not available in this build "
+            "(--stream-dir writes frames)");
     return o;
 }
 
@@ -192,20 +208,54 @@ int main(int argc, char** argv)
             usage(std::cout, argv[0]);
             return 0;
         }
+        std::unique_ptr<nbody::streaming::FrameSpool> spool;
+        std::unique_ptr<nbody::streaming::FrameSampler> sampler;
+        std::size_t frames_written = 0;
+        if (o.streaming) {
+            spool = std::make_unique<nbody::streaming::FrameSpool>(o.stream_dir);
+            sampler = std::make_unique<nbody::streaming::FrameSampler>(o.sample_rate);
+        }
+
         std::ifstream input(o.input, std::ios::binary);
         if (!input.is_open())
             throw std::runtime_error("Could not open particle input file: " + o.input);
         Particles particles(input);
         Simulation simulation(std::move(particles), o.config);
-        const std::size_t count = simulation.particles().count;
+        const DeviceParticlesView view = simulation.particles();
+        const std::size_t count = view.count;
+        std::unique_ptr<nbody::streaming::DeviceFrameWriter> writer;
+        if (spool)
+            writer = std::make_unique<nbody::streaming::DeviceFrameWriter>(
+                ConstDeviceParticlesView{view.count, view.mass, view.x, view.y, view.z,
+                                         view.vx, view.vy, view.vz, view.type},
+                o.maximum_stream_particles);
         std::cout << "Loaded " << count << " particles from " << o.input << '\n';
 
         using clock = std::chrono::steady_clock;
         const auto t0 = clock::now();
         simulation.initialize();
         const auto t1 = clock::now();
-        for (std::size_t s = 0; s < o.steps; ++s)
+        std::size_t last_captured = std::numeric_limits<std::size_t>::max();
+        const auto capture = [&](bool force) {
+            if (!sampler)
+                return;
+            const std::chrono::duration<double> since = clock::now() - t1;
+            if (!force && !sampler->isDue(since.count()))
+                return;
+            writer->write(*spool, sampler->takeSequence(), simulation.stepNumber(),
+                          simulation.time());
+            ++frames_written;
+            last_captured = simulation.stepNumber();
+        };
+        capture(false);
+        for (std::size_t s = 0; s < o.steps; ++s) {
             simulation.step();
+            capture(false);
+        }
+        if (sampler && last_captured != simulation.stepNumber())
+            capture(true);
+        if (spool)
+            (void)spool->markComplete();
         const auto t2 = clock::now();
         const std::chrono::duration<double> init = t1 - t0;
         const std::chrono::duration<double> evolution = t2 - t1;
@@ -218,6 +268,11 @@ int main(int argc, char** argv)
                   << "  average step time: "
                   << 1000.0 * evolution.count() / static_cast<double>(o.steps) << " ms\n"
                   << "  total wall time: " << total.count() << " seconds\n";
+        if (spool)
+            std::cout << "  frames written: " << frames_written << '\n'
+                      << "  particles per frame: " << writer->sampleParticleCount() << " of "
+                      << count << '\n'
+                      << "  frame spool: " << spool->directory() << '\n';
         return 0;
     } catch (const std::invalid_argument& e) {
         std::cerr << "Argument error: " << e.what() << "\n\n";

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <filesystem>
 #include <functional>
 #include <random>
 #include <stdexcept>
@@ -23,6 +24,7 @@
 #include "octree_physics.hpp"
 #include "radix_to_octree.hpp"
 #include "simulation.hpp"
+#include "streaming/device_capture.hpp"
 #include "tree_builder.hpp"
 
 namespace {
@@ -281,6 +283,55 @@ int main()
         const double com_x = (M * px[0] + m * px[1]) / (M + m);
         return std::fabs(sep - r) < 2e-3 * r && std::fabs(px[1] - xe) < 2e-2 * r &&
                std::fabs(com_x) < 1e-6 * r;
+    });
+
+    run("DeviceFrameWriter on a Simulation: sampled, quantized, typed frames", [] {
+        const std::size_t n = 3000, S = 700;
+        const Set s = uniform(n, 21);
+        std::vector<std::uint8_t> types(n);
+        for (std::size_t i = 0; i < n; ++i)
+            types[i] = static_cast<std::uint8_t>(i % 5);
+        SimulationConfig cfg{1e-3, 0.5, 1e-2};
+        cfg.gravitationalConstant = 1.0;
+        Simulation sim(Particles(s.m, s.x, s.y, s.z, s.vx, s.vy, s.vz, types), cfg);
+        sim.initialize();
+        sim.run(2);
+        const DeviceParticlesView v = sim.particles();
+        const ConstDeviceParticlesView cv{v.count, v.mass, v.x, v.y, v.z,
+                                          v.vx, v.vy, v.vz, v.type};
+        nbody::streaming::DeviceFrameWriter writer(cv, S);
+        const auto dir = std::filesystem::temp_directory_path() / "nbody-amd-shim-frames";
+        std::filesystem::remove_all(dir);
+        nbody::streaming::FrameSpool spool(dir, "shim");
+        const auto path = writer.write(spool, 4, sim.stepNumber(), sim.time());
+        const auto h = nbody::streaming::readFrameHeader(path);
+        const auto pos = nbody::streaming::readFramePositions(path);
+        const auto t = nbody::streaming::readFrameParticleTypes(path);
+        const auto px = host(v.x, n), py = host(v.y, n), pz = host(v.z, n);
+        bool ok = h.version == 3 && h.sequence == 4 && h.simulationStep == 2 &&
+                  h.particleCount == S && h.sourceParticleCount == n &&
+                  writer.sampleParticleCount() == S && writer.sourceParticleCount() == n;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (std::size_t j = 0; j < S; ++j) {
+            const std::size_t i = j * n / S;
+            const double p[3] = {px[i], py[i], pz[i]};
+            for (int c = 0; c < 3; ++c)
+                lo[c] = std::min(lo[c], p[c]), hi[c] = std::max(hi[c], p[c]);
+        }
+        for (int c = 0; c < 3; ++c)
+            ok = ok && h.bounds.minimum[c] == lo[c] && h.bounds.maximum[c] == hi[c];
+        for (std::size_t j = 0; ok && j < S; ++j) {
+            const std::size_t i = j * n / S;
+            const double p[3] = {px[i], py[i], pz[i]};
+            ok = t[j] == i % 5;
+            for (int c = 0; c < 3 && ok; ++c)
+                ok = std::fabs(pos[3 * j + c] - p[c]) <= (hi[c] - lo[c]) / 65535.0 + 1e-6;
+        }
+        const bool rejects = throws<std::invalid_argument>([&] {
+            nbody::streaming::DeviceFrameWriter w(ConstDeviceParticlesView{0}, 10);
+        });
+        std::filesystem::remove_all(dir);
+        return ok && rejects;
     });
 
     std::printf("%d failed\n", failed);

@@ -193,6 +193,27 @@ _sig("nbh_adjoint_forward", _i, _p, *([_p] * 13))
 _sig("nbh_adjoint_backward", _i, _p, *([_p] * 12))
 
 
+class FrameHeader(C.Structure):
+    _fields_ = [("version", C.c_uint32), ("header_bytes", C.c_uint32),
+                ("sequence", _u64), ("simulation_step", _u64), ("simulation_time", _d),
+                ("source_particle_count", _u64), ("particle_count", _u64),
+                ("scalar_bytes", C.c_uint32), ("components", C.c_uint32),
+                ("payload_bytes", _u64), ("total_steps", _u64),
+                ("bounds_min", _d * 3), ("bounds_max", _d * 3),
+                ("particle_type_bytes", _u64)]
+
+
+_sig("nbh_frame_capture", _i, _p, _p, _p, _p, _sz, _sz, _p, C.POINTER(_d))
+_sig("nbh_frame_sample_types", _i, _p, _p, _sz, _sz, _p)
+_sig("nbh_frame_write_quantized", _i, C.c_char_p, C.c_char_p, _u64, _u64, _d, _sz, _sz,
+     C.POINTER(_d), _p, _p, C.c_char_p, _sz)
+_sig("nbh_frame_write_float", _i, C.c_char_p, C.c_char_p, _u64, _u64, _d, _u64, _sz, _sz,
+     _p, _p, C.c_char_p, _sz)
+_sig("nbh_frame_mark_complete", _i, C.c_char_p, C.c_char_p, C.c_char_p, _sz)
+_sig("nbh_frame_read_header", _i, C.c_char_p, C.POINTER(FrameHeader))
+_sig("nbh_frame_read", _i, C.c_char_p, _sz, _p, _p, C.POINTER(_sz))
+
+
 def check(status: int) -> None:
     if status == 0:
         return
@@ -695,6 +716,175 @@ def save_particles(path, particles: dict, wide_header=True, with_types=False):
             f.write(np.ascontiguousarray(particles[k], dtype=np.float64).tobytes())
         if with_types:
             f.write(np.ascontiguousarray(particles["type"], dtype=np.uint8).tobytes())
+
+
+# ------------------------------------------------------------------ frames
+# The .nbsnap frames of graphical streaming (src/streaming/frame.hpp,
+# device_capture.hpp): sampled positions quantized to uint16 over the sample's
+# bounds, written through a durable spool directory.
+FRAME_VERSION = 3
+DEFAULT_STREAM_PARTICLES = 100000  # apps/simulation/main.cpp:36
+
+
+def _addr(t):
+    """Device address of a torch tensor, or an int address as given."""
+    if t is None:
+        return None
+    return t if isinstance(t, int) else t.data_ptr()
+
+
+def capture_frame(x, y, z, sample_count=None, count=None, ctx=None):
+    """Samples `sample_count` of `count` particles (sample j reads particle
+    j * count // sample_count) and quantizes their positions on the device.
+    Returns (uint16 device tensor of shape [S, 3], numpy bounds
+    [min x, min y, min z, max x, max y, max z])."""
+    ctx = ctx or default_context()
+    torch = _torch()
+    n = int(count if count is not None else x.numel())
+    s = n if sample_count is None else int(sample_count)
+    out = torch.empty((max(s, 1), 3), dtype=torch.uint16, device=device())
+    bounds = (_d * 6)()
+    check(lib.nbh_frame_capture(ctx.handle, _addr(x), _addr(y), _addr(z), n, s,
+                                out.data_ptr(), bounds))
+    return out[:s], np.array(bounds[:], dtype=np.float64)
+
+
+def sample_frame_types(types, sample_count=None, count=None, ctx=None):
+    """Samples particle types like capture_frame; raises DeviceError for a
+    value outside ParticleType.  Returns a uint8 device tensor."""
+    ctx = ctx or default_context()
+    torch = _torch()
+    n = int(count if count is not None else types.numel())
+    s = n if sample_count is None else int(sample_count)
+    out = torch.empty(max(s, 1), dtype=torch.uint8, device=device())
+    check(lib.nbh_frame_sample_types(ctx.handle, _addr(types), n, s, out.data_ptr()))
+    return out[:s]
+
+
+def _path_buffer():
+    return C.create_string_buffer(4096)
+
+
+class FrameSpool:
+    """FrameSpool (src/streaming/frame.hpp:77-114): publishes
+    run-<session>-frame-<20-digit sequence>.nbsnap files atomically into
+    `directory` and refuses to overwrite one.  session_id None ->
+    "<microseconds since the epoch>-<pid>"."""
+
+    def __init__(self, directory, session_id=None):
+        import re
+        import time
+        if not str(directory):
+            raise InvalidArgument("Frame spool directory cannot be empty", -1, 0)
+        if session_id is None:
+            session_id = f"{time.time_ns() // 1000}-{os.getpid()}"
+        if not session_id or not re.fullmatch(r"[A-Za-z0-9_-]+", session_id):
+            raise InvalidArgument(
+                "Frame session id may contain only letters, numbers, '-' and '_'", -1, 0)
+        self.directory = os.fspath(directory)
+        self.session_id = session_id
+        os.makedirs(self.directory, exist_ok=True)
+
+    def _args(self):
+        return os.fsencode(self.directory), self.session_id.encode()
+
+    def write_quantized_positions(self, sequence, step, time, positions, bounds,
+                                  source_count=0, types=None) -> str:
+        """Version 3 with `types`, version 2 (120-byte header) without.
+        positions: host uint16 array of shape [S, 3]."""
+        q = np.ascontiguousarray(positions, dtype=np.uint16).reshape(-1, 3)
+        t = None if types is None else np.ascontiguousarray(types, dtype=np.uint8)
+        if t is not None and t.size != q.shape[0]:
+            raise InvalidArgument("one particle type per sampled particle is required", -1, 0)
+        b = (_d * 6)(*[float(v) for v in np.asarray(bounds, dtype=np.float64).ravel()[:6]])
+        path = _path_buffer()
+        check(lib.nbh_frame_write_quantized(*self._args(), int(sequence), int(step), float(time),
+                                            q.shape[0], int(source_count), b, q.ctypes.data,
+                                            None if t is None else t.ctypes.data, path,
+                                            len(path)))
+        return os.fsdecode(path.value)
+
+    def write_typed_positions(self, sequence, step, time, total_steps, positions, types=None,
+                              source_count=0) -> str:
+        """Float32 positions: version 2 (80-byte header) with `types`,
+        version 1 without."""
+        p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        t = None if types is None else np.ascontiguousarray(types, dtype=np.uint8)
+        if t is not None and t.size != p.shape[0]:
+            raise InvalidArgument("one particle type per particle is required", -1, 0)
+        path = _path_buffer()
+        check(lib.nbh_frame_write_float(*self._args(), int(sequence), int(step), float(time),
+                                        int(total_steps), p.shape[0], int(source_count),
+                                        p.ctypes.data, None if t is None else t.ctypes.data,
+                                        path, len(path)))
+        return os.fsdecode(path.value)
+
+    def write_positions(self, sequence, step, time, positions, source_count=0) -> str:
+        return self.write_typed_positions(sequence, step, time, 0, positions, None, source_count)
+
+    def mark_complete(self) -> str:
+        path = _path_buffer()
+        check(lib.nbh_frame_mark_complete(*self._args(), path, len(path)))
+        return os.fsdecode(path.value)
+
+
+class DeviceFrameWriter:
+    """DeviceFrameWriter (src/streaming/device_capture.hpp:10-36): samples at
+    most `maximum_particles` (0 = all) of the device arrays x, y, z (torch
+    tensors or device addresses) with their particle types, validated once
+    here, and writes version-3 quantized frames."""
+
+    def __init__(self, x, y, z, types, maximum_particles=DEFAULT_STREAM_PARTICLES, count=None,
+                 ctx=None):
+        n = int(count if count is not None else x.numel())
+        if n == 0 or x is None or y is None or z is None or types is None:
+            raise InvalidArgument("Cannot capture an empty or incomplete device particle view",
+                                  -1, 0)
+        self.ctx = ctx or default_context()
+        self._xyz = (x, y, z)
+        self.source_particle_count = n
+        self.sample_particle_count = n if maximum_particles == 0 else min(n, maximum_particles)
+        self.sampled_types = sample_frame_types(types, self.sample_particle_count, n,
+                                                self.ctx).cpu().numpy()
+
+    @classmethod
+    def from_simulation(cls, sim, types, maximum_particles=DEFAULT_STREAM_PARTICLES):
+        v = sim.particles_view()
+        return cls(v.x, v.y, v.z, types, maximum_particles, count=v.count)
+
+    def capture(self):
+        """(host uint16 [S, 3] positions, bounds) of the current state."""
+        q, bounds = capture_frame(*self._xyz, self.sample_particle_count,
+                                  self.source_particle_count, self.ctx)
+        return q.cpu().numpy(), bounds
+
+    def write(self, spool: FrameSpool, sequence, step, time) -> str:
+        q, bounds = self.capture()
+        return spool.write_quantized_positions(sequence, step, time, q, bounds,
+                                               self.source_particle_count, self.sampled_types)
+
+
+def read_frame_header(path) -> dict:
+    """readFrameHeader (src/streaming/frame.cpp:507-612) as a dict."""
+    h = FrameHeader()
+    check(lib.nbh_frame_read_header(os.fsencode(str(path)), C.byref(h)))
+    out = {name: getattr(h, name) for name, _ in FrameHeader._fields_}
+    out["bounds_min"] = list(h.bounds_min)
+    out["bounds_max"] = list(h.bounds_max)
+    return out
+
+
+def read_frame(path):
+    """(float32 positions [n, 3], uint8 types [n]) of a frame of any
+    version; quantized positions decoded with the frame's bounds."""
+    h = read_frame_header(path)
+    n = int(h["particle_count"])
+    pos = np.empty((n, 3), dtype=np.float32)
+    types = np.empty(n, dtype=np.uint8)
+    got = _sz(0)
+    check(lib.nbh_frame_read(os.fsencode(str(path)), n, pos.ctypes.data, types.ctypes.data,
+                             C.byref(got)))
+    return pos, types
 
 
 # --------------------------------------------------------------- workloads

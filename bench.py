@@ -110,6 +110,38 @@ def accuracy_at(nb, torch, p, theta, args, targets, exact):
     return out
 
 
+def frame_capture_timing(nb, torch, sim, n, reps=20):
+    """Device frame capture of the simulation's current positions (after
+    the timed steps): bounds + quantize at the display sample size and at
+    the full particle count, and the whole DeviceFrameWriter.write path
+    (capture, copy to the host, atomic file publish) at the display size.
+    Algorithmic bytes per sampled particle: 24 read by the bounds pass,
+    24 read and 6 written by the quantize pass."""
+    import tempfile
+    v = sim.particles_view()
+    out = {}
+    for s in (nb.DEFAULT_STREAM_PARTICLES, n):
+        nb.capture_frame(v.x, v.y, v.z, s, n)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            nb.capture_frame(v.x, v.y, v.z, s, n)
+        dt = (time.perf_counter() - t0) / reps
+        out[f"capture_{s}"] = {"ms": round(dt * 1e3, 4),
+                               "GB_per_s": round(54 * s / dt / 1e9, 1)}
+    types = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    writer = nb.DeviceFrameWriter.from_simulation(sim, types)
+    with tempfile.TemporaryDirectory() as d:
+        spool = nb.FrameSpool(d, "bench")
+        writer.write(spool, 0, 0, 0.0)
+        t0 = time.perf_counter()
+        for k in range(1, 6):
+            writer.write(spool, k, 0, 0.0)
+        out["write_ms"] = round((time.perf_counter() - t0) / 5 * 1e3, 3)
+    out["sample"] = writer.sample_particle_count
+    return out
+
+
 def cpu_baseline(p, args):
     """The oracle (the reference algorithm in FP64 C++/OpenMP) on the same
     particles: one full rebuild plus the forces of a bounded sample of
@@ -231,6 +263,8 @@ def main():
         for th in (float(t) for t in args.thetas.split(",")):
             by_theta[str(th)] = accuracy_at(nb, torch, p, th, args, targets, exact)
 
+    frames = frame_capture_timing(nb, torch, mixed, n)
+
     trav_ms = phases["traversal"]
     achieved = FLOPS_PER_INTERACTION * interactions / (trav_ms * 1e-3) / 1e12
     line = {
@@ -266,6 +300,7 @@ def main():
                    "fp64_rel_drift": energy64, "fp64_steps": args.fp64_steps},
         "accuracy_by_theta": by_theta,
         "phase_ms": {k: round(v, 4) for k, v in phases.items()},
+        "frame_capture": frames,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),

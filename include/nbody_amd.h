@@ -380,6 +380,85 @@ int nbh_direct_sum(nbh_context* ctx, const double* d_mass, const double* d_x,
                    double softening, double gravitational_constant,
                    double* d_ax, double* d_ay, double* d_az);
 
+/* ------------------------------------------------------- frame capture */
+/* Sample j of `sample_count` (1 <= sample_count <= count) reads particle
+ * j * count / sample_count. */
+
+/* Replaces gatherPositionBounds, the host bound reduction and
+ * gatherQuantizedPositionSamples of DeviceFrameWriter::write
+ * (src/streaming/device_capture.cu:27-137,244-313).  Writes the sampled
+ * per-axis bounds {min x, min y, min z, max x, max y, max z} to h_bounds and
+ * 3 * sample_count interleaved uint16 positions to d_positions:
+ *   q = 0 when max == min,
+ *   q = (uint16)fma(clamp((p - min) / (max - min), 0, 1), 65535, 0.5) otherwise.
+ * A non-finite sampled coordinate -> NBH_RUNTIME_ERROR ("Cannot quantize a
+ * graphical frame with non-finite positions"). */
+int nbh_frame_capture(nbh_context* ctx, const double* d_x, const double* d_y,
+                      const double* d_z, size_t count, size_t sample_count,
+                      uint16_t* d_positions, double* h_bounds);
+
+/* Replaces gatherParticleTypeSamples and the type validation of the
+ * DeviceFrameWriter constructor (src/streaming/device_capture.cu:139-153,
+ * 186-214): d_types_out[j] = d_type[j * count / sample_count]; a value above
+ * ParticleType::Asteroid -> NBH_RUNTIME_ERROR ("Cannot capture a frame with
+ * invalid particle types"). */
+int nbh_frame_sample_types(nbh_context* ctx, const uint8_t* d_type, size_t count,
+                           size_t sample_count, uint8_t* d_types_out);
+
+/* The .nbsnap frame header (src/streaming/frame.hpp:25-44); the little-endian
+ * file layout is given in amsc-nbody_amd/cpp/nbody/streaming/frame.hpp. */
+typedef struct nbh_frame_header {
+    uint32_t version;      /* 1, 2 or 3 */
+    uint32_t header_bytes; /* 72, 80 or 120 (version 2), 128 */
+    uint64_t sequence;
+    uint64_t simulation_step;
+    double simulation_time;
+    uint64_t source_particle_count;
+    uint64_t particle_count;
+    uint32_t scalar_bytes; /* 4 (float32) or 2 (quantized) */
+    uint32_t components;   /* 3 */
+    uint64_t payload_bytes;
+    uint64_t total_steps;  /* 80-byte layout only */
+    double bounds_min[3];  /* quantized layouts */
+    double bounds_max[3];
+    uint64_t particle_type_bytes;
+} nbh_frame_header;
+
+/* FrameSpool (src/streaming/frame.cpp:180-505) for C callers.  session_id
+ * NULL -> "<microseconds since the epoch>-<pid>" (letters, digits, '-' and
+ * '_' only).  Each call publishes
+ * <directory>/run-<session>-frame-<20-digit sequence>.nbsnap through a ".part"
+ * file and a rename, refuses to overwrite a published frame, and copies the
+ * published path into path_out (may be NULL).  write_quantized: version 3
+ * with h_types, version 2 (120-byte header) without; bounds as written by
+ * nbh_frame_capture.  write_float: version 2 (80-byte header, total_steps)
+ * with h_types, version 1 without.  source_count 0 means `count`.  Argument
+ * errors -> NBH_INVALID_ARGUMENT, file errors -> NBH_RUNTIME_ERROR, with the
+ * reference's messages. */
+int nbh_frame_write_quantized(const char* directory, const char* session_id,
+                              uint64_t sequence, uint64_t step, double time,
+                              size_t count, size_t source_count,
+                              const double* bounds, const uint16_t* h_positions,
+                              const uint8_t* h_types, char* path_out,
+                              size_t path_capacity);
+int nbh_frame_write_float(const char* directory, const char* session_id,
+                          uint64_t sequence, uint64_t step, double time,
+                          uint64_t total_steps, size_t count, size_t source_count,
+                          const float* h_positions, const uint8_t* h_types,
+                          char* path_out, size_t path_capacity);
+/* FrameSpool::markComplete: publishes run-<session>.complete holding the
+ * session id and a newline; idempotent. */
+int nbh_frame_mark_complete(const char* directory, const char* session_id,
+                            char* path_out, size_t path_capacity);
+/* readFrameHeader, readFramePositions, readFrameParticleTypes
+ * (src/streaming/frame.cpp:507-720): every layout, validated against the
+ * file size; quantized positions are decoded to float32 with the bounds.
+ * h_types (may be NULL) reads as ParticleType::Unknown for frames without
+ * a type block. */
+int nbh_frame_read_header(const char* path, nbh_frame_header* out);
+int nbh_frame_read(const char* path, size_t capacity, float* h_positions,
+                   uint8_t* h_types, size_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,9 @@
 // tests/unit/cuda/*.cu and tests/integration/cuda/*.cu, std::stable_sort and
 // std::minmax_element equivalence, brute-force recursive octrees, the
 // direct-sum limit at theta = 0, and closed forms for the integrator.
+// Frame capture (tests/test_frames.py): hand-computed sample indices, exact
+// rational arithmetic for the bounds and for every quantized value, and the
+// endpoint values of the reference's frame test.
 
 #include <algorithm>
 #include <cmath>
@@ -1294,6 +1297,58 @@ auto with_sim(SimHandle* h, F&& f)
     return f(h->s21);
 }
 
+// ---------------------------------------------------------------- frames
+// src/streaming/device_capture.cu:27-137 (gatherPositionBounds,
+// quantizePosition, gatherQuantizedPositionSamples) and :139-153
+// (gatherParticleTypeSamples), with the host reduction of write() at
+// :252-294.  Sample j of s reads particle j * n / s (integer division).
+// Bounds are the per-axis min/max over the sampled positions; the reference
+// reduces with fmin/fmax, which leave the choice between -0 and +0 open, and
+// this restatement fixes it: -0 orders below +0.  quantizePosition's
+// `normalized * 65535 + 0.5` is contracted to one fma by nvcc's default
+// --fmad, so the fma is written out here.
+std::size_t frame_source(std::size_t j, std::size_t n, std::size_t s)
+{
+    return (j * n) / s;
+}
+
+bool frame_less(double a, double b)
+{
+    return a < b || (a == b && std::signbit(a) && !std::signbit(b));
+}
+
+void frame_bounds(const double* x, const double* y, const double* z, std::size_t n,
+                  std::size_t s, double* bounds)
+{
+    const double inf = std::numeric_limits<double>::infinity();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    for (std::size_t j = 0; j < s; ++j) {
+        const std::size_t i = frame_source(j, n, s);
+        const double p[3] = {x[i], y[i], z[i]};
+        if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]))
+            fail(RUNTIME_ERROR, "Cannot quantize a graphical frame with non-finite positions");
+        for (int c = 0; c < 3; ++c) {
+            if (frame_less(p[c], lo[c]))
+                lo[c] = p[c];
+            if (frame_less(hi[c], p[c]))
+                hi[c] = p[c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        bounds[c] = lo[c];
+        bounds[3 + c] = hi[c];
+    }
+}
+
+std::uint16_t frame_quantize(double p, double lo, double hi)
+{
+    if (hi == lo)
+        return 0;
+    double t = (p - lo) / (hi - lo);
+    t = std::fmin(1.0, std::fmax(0.0, t));
+    return static_cast<std::uint16_t>(std::fma(t, 65535.0, 0.5));
+}
+
 } // namespace
 
 // =====================================================================
@@ -1709,6 +1764,35 @@ int orc_sim_structure(void* h, void* keys, std::uint32_t* idx, int* children,
                 std::memcpy(cz, s.cz.data(), M * sizeof(double));
             return 0;
         });
+    });
+}
+
+// Frame capture (device_capture.cu:27-153,244-313): bounds[6] and the
+// 3*s interleaved quantized positions of the sample; types (optional) are
+// the sampled type codes, any above 4 (ParticleType::Asteroid) an error.
+int orc_frame_capture(const double* x, const double* y, const double* z,
+                      const std::uint8_t* type, std::size_t n, std::size_t s,
+                      double* bounds, std::uint16_t* q, std::uint8_t* types)
+{
+    return guarded([&] {
+        if (n == 0 || s == 0 || s > n)
+            fail(INVALID_ARGUMENT, "frame sample count must be in [1, particle count]");
+        if (type && types) {
+            for (std::size_t j = 0; j < s; ++j) {
+                types[j] = type[frame_source(j, n, s)];
+                if (types[j] > 4)
+                    fail(RUNTIME_ERROR, "Cannot capture a frame with invalid particle types");
+            }
+        }
+        if (!x)
+            return;
+        frame_bounds(x, y, z, n, s, bounds);
+        for (std::size_t j = 0; j < s; ++j) {
+            const std::size_t i = frame_source(j, n, s);
+            q[3 * j + 0] = frame_quantize(x[i], bounds[0], bounds[3]);
+            q[3 * j + 1] = frame_quantize(y[i], bounds[1], bounds[4]);
+            q[3 * j + 2] = frame_quantize(z[i], bounds[2], bounds[5]);
+        }
     });
 }
 

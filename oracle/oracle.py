@@ -97,6 +97,26 @@ def morton_keys(x, y, z, box, bits):
     return keys, idx
 
 
+def frame_capture(x, y, z, sample_count=None, types=None):
+    """Sampled bounds [6] and quantized positions [S, 3] (uint16) of a frame,
+    plus the sampled types when `types` is given (device_capture.cu:27-153)."""
+    x, y, z = _a(x), _a(y), _a(z)
+    n = len(x)
+    s = n if sample_count is None else int(sample_count)
+    bounds = np.zeros(6)
+    q = np.zeros((max(s, 1), 3), dtype=np.uint16)
+    t_in = None if types is None else _a(types, np.uint8)
+    t_out = None if types is None else np.zeros(max(s, 1), dtype=np.uint8)
+    _chk(lib().orc_frame_capture(_p(x), _p(y), _p(z), _p(t_in), C.c_size_t(n),
+                                 C.c_size_t(s), _p(bounds), _p(q), _p(t_out)))
+    return bounds, q[:s], (None if t_out is None else t_out[:s])
+
+
+def frame_sample_indices(n, s):
+    """Source particle of each sample: j * n // s (device_capture.cu:50-51)."""
+    return (np.arange(s, dtype=np.uint64) * np.uint64(n)) // np.uint64(s)
+
+
 def leaf_groups(sorted_keys, capacity=None):
     k = np.ascontiguousarray(sorted_keys)
     n = len(k)

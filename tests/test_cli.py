@@ -54,6 +54,19 @@ def test_missing_input_file(tmp_path):
     assert r.returncode == 1 and "Could not open particle input file" in r.stderr
 
 
-def test_streaming_is_an_argument_error():
+def test_stream_server_is_an_argument_error():
     r = run("--input", "x.bin", "--time-step", "0.25", "--steps", "2", "--cluster")
-    assert r.returncode == 2 and "streaming" in r.stderr
+    assert r.returncode == 2 and "stream server is not available" in r.stderr
+    r = run("--input", "x.bin", "--time-step", "0.25", "--steps", "2", "--stream-dir", "")
+    assert r.returncode == 2 and "--stream-dir must not be empty" in r.stderr
+    r = run("--input", "x.bin", "--time-step", "0.25", "--steps", "2", "--sample-rate", "0")
+    assert r.returncode == 2 and "--sample-rate must be positive" in r.stderr
+
+
+def test_stream_spool_is_created_before_loading(tmp_path):
+    # the spool directory exists before the input is opened (and fails)
+    spool = tmp_path / "frames"
+    r = run("--input", str(tmp_path / "missing.bin"), "--time-step", "0.25", "--steps", "2",
+            "--stream-dir", str(spool))
+    assert r.returncode == 1 and "Could not open particle input file" in r.stderr
+    assert spool.is_dir() and not any(spool.iterdir())

@@ -20,3 +20,4 @@ def test_cpp_drop_in_layer():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
+    assert r.stdout.count("PASS") == 9
