@@ -6,9 +6,10 @@
 // Bounds: the per-axis minimum and maximum of the finite sampled positions.
 // Each value is reduced as an order-preserving 64-bit key, so the result does
 // not depend on the reduction order.  The key order is total: -0 < +0, and
-// NaN never enters it.  One pass reduces a wave with shuffles, then a block
-// through LDS, then issues one atomicMin / atomicMax per axis per block.  A
-// non-finite sample sets a flag instead.
+// NaN never enters it.  The bounds pass reduces a wave with shuffles, then a
+// block through LDS, and writes one record per block: six keys and a flag
+// for non-finite samples.  A one-block pass folds the block records.  No
+// atomics: same-address atomics from thousands of blocks serialise in L2.
 //
 // Quantization:
 //   q = 0                                               when max == min,
@@ -34,6 +35,7 @@ namespace {
 
 constexpr unsigned kCaptureThreads = 256;
 constexpr unsigned kBoundsMaxBlocks = 2048;
+static_assert(kBoundsMaxBlocks % kCaptureThreads == 0, "finish pass reads whole rounds");
 constexpr int kMaxParticleType = 4; // ParticleType::Asteroid (src/particle_type.hpp:7-18)
 
 __device__ __forceinline__ unsigned long long order_key(double v)
@@ -62,69 +64,115 @@ __device__ __forceinline__ uint16_t quantize(double p, double lo, double hi)
     return static_cast<uint16_t>(fma(t, 65535.0, 0.5));
 }
 
-// keys[0..2] = minimum keys (start at the top of the order), keys[3..5] =
-// maximum keys (start at the bottom), *bad = 0.
-__global__ void frame_bounds_init_kernel(unsigned long long* __restrict__ keys,
-                                         int* __restrict__ bad)
+constexpr unsigned kWaves = kCaptureThreads / kWave;
+
+// Block-level reduction of six keys and a flag.  red[c][wave] holds the wave
+// results; returns the block result in threads 0..6 (key c, or the flag as 6).
+__device__ __forceinline__ unsigned long long block_fold(unsigned long long v[6], bool bad,
+                                                         unsigned long long (*red)[kWaves])
 {
-    const unsigned t = threadIdx.x;
-    if (t < 6)
-        keys[t] = t < 3 ? ~0ull : 0ull;
-    if (t == 0)
-        *bad = 0;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const unsigned long long o = __shfl_xor(v[c], off);
+            v[c] = c < 3 ? min(v[c], o) : max(v[c], o);
+        }
+    }
+    const bool wave_bad = __any(bad);
+    const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            red[c][wave] = v[c];
+        red[6][wave] = wave_bad ? 1ull : 0ull;
+    }
+    __syncthreads();
+    unsigned long long r = 0;
+    if (threadIdx.x < 7) {
+        const unsigned c = threadIdx.x;
+        r = red[c][0];
+        for (unsigned w = 1; w < kWaves; ++w)
+            r = c < 3 ? min(r, red[c][w]) : (c < 6 ? max(r, red[c][w]) : (r | red[c][w]));
+    }
+    return r;
 }
 
+// Per-block record: blocks[8 b + c] = minimum keys (c < 3), maximum keys
+// (3 <= c < 6) and the non-finite flag (c = 6) of block b's samples.
 __global__ __launch_bounds__(kCaptureThreads) void frame_bounds_kernel(
     const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, size_t n, size_t s,
-    unsigned long long* __restrict__ keys, int* __restrict__ bad)
+    unsigned long long* __restrict__ blocks)
 {
-    unsigned long long lo0 = ~0ull, lo1 = ~0ull, lo2 = ~0ull;
-    unsigned long long hi0 = 0, hi1 = 0, hi2 = 0;
+    unsigned long long v[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
     bool invalid = false;
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-    for (size_t j = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < s;
-         j += stride) {
-        const size_t i = source_of(j, n, s);
-        const double px = x[i], py = y[i], pz = z[i];
-        if (isfinite(px) && isfinite(py) && isfinite(pz)) {
-            const unsigned long long kx = order_key(px), ky = order_key(py),
-                                     kz = order_key(pz);
-            lo0 = min(lo0, kx), lo1 = min(lo1, ky), lo2 = min(lo2, kz);
-            hi0 = max(hi0, kx), hi1 = max(hi1, ky), hi2 = max(hi2, kz);
-        } else {
-            invalid = true;
+    // Four samples per lane per round, loads first, so that twelve loads
+    // are in flight per lane.
+    constexpr int kUnroll = 4;
+    for (size_t j0 = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j0 < s;
+         j0 += kUnroll * stride) {
+        double px[kUnroll], py[kUnroll], pz[kUnroll];
+        bool in[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const size_t j = j0 + u * stride;
+            in[u] = j < s;
+            const size_t i = in[u] ? source_of(j, n, s) : 0;
+            px[u] = x[i], py[u] = y[i], pz[u] = z[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (!in[u])
+                continue;
+            if (isfinite(px[u]) && isfinite(py[u]) && isfinite(pz[u])) {
+                const unsigned long long kx = order_key(px[u]), ky = order_key(py[u]),
+                                         kz = order_key(pz[u]);
+                v[0] = min(v[0], kx), v[1] = min(v[1], ky), v[2] = min(v[2], kz);
+                v[3] = max(v[3], kx), v[4] = max(v[4], ky), v[5] = max(v[5], kz);
+            } else {
+                invalid = true;
+            }
         }
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        lo0 = min(lo0, __shfl_xor(lo0, off));
-        lo1 = min(lo1, __shfl_xor(lo1, off));
-        lo2 = min(lo2, __shfl_xor(lo2, off));
-        hi0 = max(hi0, __shfl_xor(hi0, off));
-        hi1 = max(hi1, __shfl_xor(hi1, off));
-        hi2 = max(hi2, __shfl_xor(hi2, off));
+    __shared__ unsigned long long red[7][kWaves];
+    const unsigned long long r = block_fold(v, invalid, red);
+    if (threadIdx.x < 7)
+        blocks[8 * static_cast<size_t>(blockIdx.x) + threadIdx.x] = r;
+}
+
+// One block: folds `nblocks` block records into keys[0..5] and keys[6].
+__global__ __launch_bounds__(kCaptureThreads) void frame_bounds_finish_kernel(
+    const unsigned long long* __restrict__ blocks, unsigned nblocks,
+    unsigned long long* __restrict__ keys)
+{
+    unsigned long long v[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+    bool invalid = false;
+    // nblocks <= kBoundsMaxBlocks = 8 * kCaptureThreads: at most eight
+    // records per thread, all loaded before they are folded.
+    constexpr int kPer = kBoundsMaxBlocks / kCaptureThreads;
+    unsigned long long rec[kPer][7];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const unsigned b = threadIdx.x + k * kCaptureThreads;
+        const unsigned long long* r = blocks + 8 * static_cast<size_t>(b < nblocks ? b : 0);
+#pragma unroll
+        for (int c = 0; c < 7; ++c)
+            rec[k][c] = r[c];
     }
-    const bool wave_bad = __any(invalid);
-    constexpr unsigned kWaves = kCaptureThreads / kWave;
-    __shared__ unsigned long long red[6][kWaves];
-    const unsigned lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    if (lane == 0) {
-        red[0][wave] = lo0, red[1][wave] = lo1, red[2][wave] = lo2;
-        red[3][wave] = hi0, red[4][wave] = hi1, red[5][wave] = hi2;
-        if (wave_bad)
-            atomicOr(bad, 1);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (threadIdx.x + k * kCaptureThreads >= nblocks)
+            continue;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            v[c] = c < 3 ? min(v[c], rec[k][c]) : max(v[c], rec[k][c]);
+        invalid = invalid || rec[k][6] != 0ull;
     }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        const unsigned c = threadIdx.x;
-        unsigned long long v = red[c][0];
-        for (unsigned w = 1; w < kWaves; ++w)
-            v = c < 3 ? min(v, red[c][w]) : max(v, red[c][w]);
-        if (c < 3)
-            atomicMin(keys + c, v);
-        else
-            atomicMax(keys + c, v);
-    }
+    __shared__ unsigned long long red[7][kWaves];
+    const unsigned long long r = block_fold(v, invalid, red);
+    if (threadIdx.x < 7)
+        keys[threadIdx.x] = r;
 }
 
 // One thread per pair of samples.  WIDE: the six quantized values of a pair
@@ -197,17 +245,20 @@ void check_sample(size_t count, size_t sample_count)
 
 } // namespace
 
-size_t frame_capture_scratch_bytes() { return 6 * sizeof(unsigned long long) + 64; }
+// keys: 8 words (six keys, the flag, padding), then the block records.
+size_t frame_capture_scratch_bytes() { return (8 + 8 * size_t{kBoundsMaxBlocks}) * 8; }
 
 void launch_frame_capture(hipStream_t st, const double* x, const double* y,
                           const double* z, size_t n, size_t s, uint16_t* out,
-                          unsigned long long* keys, int* bad)
+                          unsigned long long* keys)
 {
-    hipLaunchKernelGGL(frame_bounds_init_kernel, dim3(1), dim3(64), 0, st, keys, bad);
     const unsigned bb = static_cast<unsigned>(
         std::min<size_t>(kBoundsMaxBlocks, blocks_for(s, kCaptureThreads)));
+    unsigned long long* blocks = keys + 8;
     hipLaunchKernelGGL(frame_bounds_kernel, dim3(bb), dim3(kCaptureThreads), 0, st, x, y,
-                       z, n, s, keys, bad);
+                       z, n, s, blocks);
+    hipLaunchKernelGGL(frame_bounds_finish_kernel, dim3(1), dim3(kCaptureThreads), 0, st,
+                       blocks, bb, keys);
     const unsigned qb = blocks_for((s + 1) / 2, kCaptureThreads);
     if (reinterpret_cast<uintptr_t>(out) % 4 == 0)
         hipLaunchKernelGGL(frame_quantize_kernel<true>, dim3(qb), dim3(kCaptureThreads), 0,
@@ -242,22 +293,17 @@ extern "C" int nbh_frame_capture(nbh_context* ctx, const double* d_x, const doub
         need(d_positions && h_bounds, NBH_INVALID_ARGUMENT,
              "frame output pointers must not be null");
         check_sample(count, sample_count);
-        char* scratch =
-            static_cast<char*>(ctx->ws.get(SLOT_MISC, frame_capture_scratch_bytes()));
-        auto* keys = reinterpret_cast<unsigned long long*>(scratch);
-        int* bad = reinterpret_cast<int*>(scratch + 6 * sizeof(unsigned long long));
+        auto* keys = static_cast<unsigned long long*>(
+            ctx->ws.get(SLOT_MISC, frame_capture_scratch_bytes()));
         launch_frame_capture(ctx->stream, d_x, d_y, d_z, count, sample_count, d_positions,
-                             keys, bad);
+                             keys);
         // Readback: the six keys and the flag through the pinned words.
-        NBH_HIP(hipMemcpyAsync(ctx->host_pinned, scratch,
-                               6 * sizeof(unsigned long long) + sizeof(int),
+        NBH_HIP(hipMemcpyAsync(ctx->host_pinned, keys, 7 * sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, ctx->stream));
         NBH_HIP(hipStreamSynchronize(ctx->stream));
-        unsigned long long k[6];
-        int flag = 0;
+        unsigned long long k[7];
         std::memcpy(k, ctx->host_pinned, sizeof(k));
-        std::memcpy(&flag, reinterpret_cast<char*>(ctx->host_pinned) + sizeof(k), sizeof(int));
-        if (flag != 0)
+        if (k[6] != 0)
             fail(NBH_RUNTIME_ERROR, "Cannot quantize a graphical frame with non-finite positions");
         for (int c = 0; c < 6; ++c)
             h_bounds[c] = frame_key_value(k[c]);
