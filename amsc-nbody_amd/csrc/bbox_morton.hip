@@ -2,10 +2,11 @@
 // Morton-order particle gather.
 //
 // Bounding box: one pass, FP64 min/max per lane, wave64 shuffles, LDS across
-// the 4 waves of a block, per-block partials, and the last block (arrival
-// counter) finishes the box on the device with the reference's arithmetic
+// the 4 waves of a block and one record per block; a one-block pass folds the
+// records and finishes the box on the device with the reference's arithmetic
 // (globalbounding.cu:122-136: centre = (min+max)*0.5, side = 1.1*max range,
-// side 0 -> 1).  The kick-drift variant applies the first half kick and the
+// side 0 -> 1).  No arrival counter: a fence and a same-address atomic per
+// block serialise across thousands of blocks.  The kick-drift variant applies the first half kick and the
 // drift of the leapfrog step (integrator.cu:51-83) in the same pass, so the
 // new positions are reduced while they are in registers.
 //
@@ -22,7 +23,8 @@ namespace nbh {
 namespace {
 
 constexpr int kBoxThreads = 256;
-constexpr int kBoxMaxBlocks = 1024;
+constexpr int kBoxMaxBlocks = 2048;
+static_assert(kBoxMaxBlocks % kBoxThreads == 0, "finish pass reads whole rounds");
 
 struct Extent {
     double lo[3];
@@ -67,33 +69,35 @@ __device__ __forceinline__ void block_reduce_store(Extent e, double* partials)
     }
 }
 
-// The last block to arrive reduces all partials and writes the box.
-__device__ __forceinline__ void finish_box(const double* partials,
-                                           unsigned* counter, double* box,
-                                           int* err)
+// One block: folds `nblocks` block records (all loaded before they are
+// folded) and writes the box.
+__global__ __launch_bounds__(kBoxThreads) void box_finish_kernel(
+    const double* __restrict__ partials, unsigned nblocks, double* __restrict__ box,
+    int* __restrict__ err)
 {
-    __shared__ bool s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned done = atomicAdd(counter, 1u);
-        s_last = (done == gridDim.x - 1);
+    constexpr int kPer = kBoxMaxBlocks / kBoxThreads;
+    double rec[kPer][7];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const unsigned b = threadIdx.x + k * kBoxThreads;
+        const double* p = partials + 8 * static_cast<size_t>(b < nblocks ? b : 0);
+#pragma unroll
+        for (int c = 0; c < 7; ++c)
+            rec[k][c] = p[c];
     }
-    __syncthreads();
-    if (!s_last)
-        return;
-    __threadfence();
     const double inf = __builtin_inf();
     Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false};
     double bad = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-        const double* p = partials + b * 8;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (threadIdx.x + k * kBoxThreads >= nblocks)
+            continue;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            e.lo[a] = fmin(e.lo[a], p[a]);
-            e.hi[a] = fmax(e.hi[a], p[3 + a]);
+            e.lo[a] = fmin(e.lo[a], rec[k][a]);
+            e.hi[a] = fmax(e.hi[a], rec[k][3 + a]);
         }
-        bad = fmax(bad, p[6]);
+        bad = fmax(bad, rec[k][6]);
     }
     e.bad = bad != 0.0;
     __shared__ double s_red[4][7];
@@ -146,15 +150,13 @@ __device__ __forceinline__ void finish_box(const double* partials,
         box[3] = side;
         if (r[6] != 0.0)
             atomicCAS(err, 0, 1);
-        *counter = 0u; // ready for the next launch
     }
 }
 
 __global__ __launch_bounds__(kBoxThreads) void bbox_kernel(
     const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, size_t n, double* __restrict__ partials,
-    unsigned* counter, double* box, int* err, const double* __restrict__ m,
-    double4* __restrict__ pos4)
+    const double* __restrict__ m, double4* __restrict__ pos4)
 {
     const double inf = __builtin_inf();
     Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false};
@@ -173,7 +175,6 @@ __global__ __launch_bounds__(kBoxThreads) void bbox_kernel(
         e.hi[2] = fmax(e.hi[2], pz);
     }
     block_reduce_store(e, partials);
-    finish_box(partials, counter, box, err);
 }
 
 // First half kick + drift (integrator.cu:51-83: v += (dt/2) a; x += dt v,
@@ -184,8 +185,8 @@ __global__ __launch_bounds__(kBoxThreads) void kick_drift_bbox_kernel(
     double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
     const double* __restrict__ ax, const double* __restrict__ ay,
     const double* __restrict__ az, size_t n, double dt,
-    double* __restrict__ partials, unsigned* counter, double* box, int* err,
-    const double* __restrict__ m, double4* __restrict__ pos4)
+    double* __restrict__ partials, const double* __restrict__ m,
+    double4* __restrict__ pos4)
 {
     const double half = 0.5 * dt;
     const double inf = __builtin_inf();
@@ -216,12 +217,11 @@ __global__ __launch_bounds__(kBoxThreads) void kick_drift_bbox_kernel(
         e.hi[2] = fmax(e.hi[2], pz);
     }
     block_reduce_store(e, partials);
-    finish_box(partials, counter, box, err);
 }
 
 unsigned box_blocks(size_t n)
 {
-    unsigned b = blocks_for(n, kBoxThreads * 4);
+    unsigned b = blocks_for(n, kBoxThreads * 2);
     return b > kBoxMaxBlocks ? kBoxMaxBlocks : b;
 }
 
@@ -362,10 +362,11 @@ __global__ __launch_bounds__(256) void gather_kernel(
 
 void launch_bbox(nbh_context* ctx, const double* x, const double* y,
                  const double* z, size_t n, double* d_box, double* partials,
-                 unsigned* counter, int* err, const double* m, double4* pos4)
+                 int* err, const double* m, double4* pos4)
 {
-    bbox_kernel<<<box_blocks(n), kBoxThreads, 0, ctx->stream>>>(
-        x, y, z, n, partials, counter, d_box, err, m, pos4);
+    const unsigned nb = box_blocks(n);
+    bbox_kernel<<<nb, kBoxThreads, 0, ctx->stream>>>(x, y, z, n, partials, m, pos4);
+    box_finish_kernel<<<1, kBoxThreads, 0, ctx->stream>>>(partials, nb, d_box, err);
     NBH_HIP(hipGetLastError());
 }
 
@@ -373,13 +374,13 @@ void launch_kick_drift_bbox(hipStream_t s, double* x, double* y, double* z,
                             double* vx, double* vy, double* vz,
                             const double* ax, const double* ay,
                             const double* az, size_t n, double dt,
-                            double* d_box, double* partials,
-                            unsigned* counter, int* err, const double* m,
-                            double4* pos4)
+                            double* d_box, double* partials, int* err,
+                            const double* m, double4* pos4)
 {
-    kick_drift_bbox_kernel<<<box_blocks(n), kBoxThreads, 0, s>>>(
-        x, y, z, vx, vy, vz, ax, ay, az, n, dt, partials, counter, d_box,
-        err, m, pos4);
+    const unsigned nb = box_blocks(n);
+    kick_drift_bbox_kernel<<<nb, kBoxThreads, 0, s>>>(x, y, z, vx, vy, vz, ax, ay, az, n,
+                                                       dt, partials, m, pos4);
+    box_finish_kernel<<<1, kBoxThreads, 0, s>>>(partials, nb, d_box, err);
     NBH_HIP(hipGetLastError());
 }
 

@@ -165,14 +165,10 @@ int nbh_bbox_compute(nbh_context* ctx, const double* d_x, const double* d_y,
              "Cannot compute a bounding box for an empty particle set.");
         need(d_x && d_y && d_z && d_box, NBH_INVALID_ARGUMENT,
              "Bounding-box pointers must not be null");
-        char* misc = static_cast<char*>(
-            ctx->ws.get(SLOT_MISC, bbox_partial_doubles() * 8 + 256));
-        double* partials = reinterpret_cast<double*>(misc);
-        unsigned* counter =
-            reinterpret_cast<unsigned*>(misc + bbox_partial_doubles() * 8);
+        double* partials = static_cast<double*>(
+            ctx->ws.get(SLOT_MISC, bbox_partial_doubles() * 8));
         clear_errors(ctx);
-        launch_fill_u32(ctx->stream, counter, 0u, 1);
-        launch_bbox(ctx, d_x, d_y, d_z, count, d_box, partials, counter,
+        launch_bbox(ctx, d_x, d_y, d_z, count, d_box, partials,
                     error_words(ctx) + ERR_BBOX);
         if (read_error(ctx, ERR_BBOX) != 0)
             fail(NBH_DOMAIN_ERROR,

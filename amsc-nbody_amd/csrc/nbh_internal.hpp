@@ -164,14 +164,12 @@ size_t bbox_partial_doubles();
 // every particle (original order) that the Morton gather reads.
 void launch_bbox(nbh_context* ctx, const double* x, const double* y,
                  const double* z, size_t n, double* d_box, double* partials,
-                 unsigned* counter, int* err, const double* m = nullptr,
-                 double4* pos4 = nullptr);
+                 int* err, const double* m = nullptr, double4* pos4 = nullptr);
 void launch_kick_drift_bbox(hipStream_t s, double* x, double* y, double* z,
                             double* vx, double* vy, double* vz,
                             const double* ax, const double* ay,
                             const double* az, size_t n, double dt,
-                            double* d_box, double* partials,
-                            unsigned* counter, int* err,
+                            double* d_box, double* partials, int* err,
                             const double* m = nullptr, double4* pos4 = nullptr);
 template <int BITS>
 void launch_encode(hipStream_t s, const double* x, const double* y,

@@ -123,7 +123,7 @@ struct nbh_sim {
     // small device scalars
     nbh::DevBuf<double> box;      // 4
     nbh::DevBuf<double> partials; // bbox partials
-    nbh::DevBuf<int> scalars;     // [0] counter [1] ngroups [2] nnodes [3] wcount
+    nbh::DevBuf<int> scalars;     // [1] ngroups [2] nnodes [3] wcount ([0] unused)
     nbh::DevBuf<unsigned long long> counters;
 
     // octree (capacity grows)

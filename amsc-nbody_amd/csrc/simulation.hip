@@ -131,18 +131,17 @@ void rebuild(nbh_sim* s, bool kick_drift)
     s->mark(PH_KICK_DRIFT_BBOX);
     s->views_fresh = false;
     launch_fill_u32(st, s->zero.p, 0u, s->zero_words);
-    unsigned* counter = reinterpret_cast<unsigned*>(s->scalars.p);
     if (kick_drift) {
         auto& c = s->acc[s->cur];
         launch_kick_drift_bbox(st, s->x.p, s->y.p, s->z.p, s->vx.p, s->vy.p,
                                s->vz.p, c[0].p, c[1].p, c[2].p, n,
                                s->cfg.time_step, s->box.p, s->partials.p,
-                               counter, s->errors + ERR_BBOX, s->m.p, s->pos4.p);
+                               s->errors + ERR_BBOX, s->m.p, s->pos4.p);
     } else {
         nbh_context tmp;
         tmp.stream = st;
         launch_bbox(&tmp, s->x.p, s->y.p, s->z.p, n, s->box.p, s->partials.p,
-                    counter, s->errors + ERR_BBOX, s->m.p, s->pos4.p);
+                    s->errors + ERR_BBOX, s->m.p, s->pos4.p);
     }
     s->mark(PH_ENCODE);
     K* keys = reinterpret_cast<K*>(s->keys.p);
