@@ -34,9 +34,11 @@ template <>
 struct SortTile<uint32_t> {
     static constexpr int items = 16;
 };
+// 4096-key tiles for both widths: fewer tiles mean fewer tickets and shorter
+// look-backs (8 items: +14 % time, 12: +10 %, 20-24: within 1 % at 4M keys).
 template <>
 struct SortTile<uint64_t> {
-    static constexpr int items = 12;
+    static constexpr int items = 16;
 };
 
 template <typename K>
@@ -44,6 +46,9 @@ constexpr int tile_size()
 {
     return kSortThreads * SortTile<K>::items;
 }
+// The look-back status is sized from the uint64 tile count.
+static_assert(tile_size<uint32_t>() >= tile_size<uint64_t>(),
+              "uint32 tiles must not outnumber uint64 tiles");
 
 template <typename K>
 __global__ __launch_bounds__(256) void histogram_kernel(const K* __restrict__ keys,
@@ -209,8 +214,7 @@ size_t sort_tiles(size_t n)
 }
 
 // Words of look-back status + tickets needed for `passes` passes over n keys
-// of the widest key type (sized for uint32 tiles, which are the larger
-// count).
+// (uint32 and uint64 tiles hold the same number of keys).
 size_t sort_status_words(size_t n, int passes)
 {
     const size_t tiles = (n + tile_size<uint64_t>() - 1) / tile_size<uint64_t>();

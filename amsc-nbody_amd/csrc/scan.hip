@@ -14,7 +14,10 @@
 //
 // Tiles take tickets in launch order; a tile publishes its aggregate, looks
 // back over earlier tiles (64-bit status words: 2 flag bits + 62 payload
-// bits, agent-scope atomics) and publishes its inclusive prefix.
+// bits, agent-scope atomics) and publishes its inclusive prefix.  The
+// look-back is wave-wide: the 64 lanes of the first wave read 64 preceding
+// status words at once and reduce them up to the nearest inclusive one, so a
+// chain of aggregate-only predecessors costs one round trip per 64 tiles.
 #include "device_utils.hpp"
 #include "nbh_internal.hpp"
 
@@ -50,6 +53,14 @@ __device__ __forceinline__ long long shfl_up(long long v, int o)
 {
     return __shfl_up(v, o);
 }
+__device__ __forceinline__ SumMax shfl_xor(SumMax v, int o)
+{
+    return {__shfl_xor(v.a, o), __shfl_xor(v.b, o)};
+}
+__device__ __forceinline__ long long shfl_xor(long long v, int o)
+{
+    return __shfl_xor(v, o);
+}
 __device__ __forceinline__ unsigned long long pack(SumMax v)
 {
     return (static_cast<unsigned long long>(v.a) << 31) |
@@ -71,7 +82,8 @@ __device__ __forceinline__ void unpack(unsigned long long w, long long& v)
 
 // Scan of one tile whose thread-inclusive values are in v[0..kScanItems).
 // On return v[] holds the global inclusive values.  `identity` must be the
-// op's identity.  Thread 0 performs the look-back.
+// op's identity, packed as payload 0.  The first wave performs the
+// look-back.
 template <typename T>
 __device__ void tile_scan(T (&v)[kScanItems], T identity, uint32_t tile,
                           unsigned long long* status)
@@ -109,30 +121,47 @@ __device__ void tile_scan(T (&v)[kScanItems], T identity, uint32_t tile,
     if (lane == 0)
         excl = identity;
     excl = combine(wave_prefix, excl);
-    if (threadIdx.x == 0) {
+    if (wave == 0) {
         T prefix = identity;
         if (tile == 0) {
-            store_agent(status, kInc | pack(tile_total));
+            if (lane == 0)
+                store_agent(status, kInc | pack(tile_total));
         } else {
-            store_agent(status + tile, kAgg | pack(tile_total));
+            if (lane == 0)
+                store_agent(status + tile, kAgg | pack(tile_total));
+            // Window of tiles j, j-1, ..., j-63 (lane k reads tile j-k).
+            // Tiles below 0 read as inclusive identities (payload 0).
             int j = static_cast<int>(tile) - 1;
             while (true) {
-                const unsigned long long w = load_agent(status + j);
+                const int idx = j - lane;
+                const unsigned long long w = idx >= 0 ? load_agent(status + idx) : kInc;
                 const unsigned long long flag = w & ~kPayload;
-                if (flag == 0ull) {
+                const unsigned long long inc = __ballot(flag == kInc);
+                const unsigned long long pending = __ballot(flag == 0ull);
+                // lanes 0..first inclusive lane (all lanes when none is)
+                const int first = inc ? __ffsll(static_cast<long long>(inc)) - 1 : 63;
+                const unsigned long long need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+                if ((pending & need) != 0ull) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
                 T val;
                 unpack(w, val);
+                if (lane > first)
+                    val = identity;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1)
+                    val = combine(val, shfl_xor(val, o));
                 prefix = combine(val, prefix);
-                if (flag == kInc)
+                if (inc != 0ull)
                     break;
-                --j;
+                j -= 64;
             }
-            store_agent(status + tile, kInc | pack(combine(prefix, tile_total)));
+            if (lane == 0)
+                store_agent(status + tile, kInc | pack(combine(prefix, tile_total)));
         }
-        s_prefix = prefix;
+        if (lane == 0)
+            s_prefix = prefix;
     }
     __syncthreads();
     const T before = combine(s_prefix, excl);
