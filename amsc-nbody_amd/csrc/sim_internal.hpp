@@ -22,6 +22,7 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          hipEvent_t between);
 // ints of the level-pass bookkeeping array (level starts + per-block counts)
 size_t level_info_ints();
+void launch_walk_sentinel(hipStream_t s, int4* w32node, int4* w64link, int cap);
 template <int BITS>
 void launch_octree_views(hipStream_t s, const nbh_octree_view& o,
                          const double4* sum4, const double* box);

@@ -116,6 +116,10 @@ void ensure_octree(nbh_sim* s, size_t m)
         if (s->cfg.theta >= kContainsTheta)
             s->w64center.alloc(cap);
     }
+    if (cap > 0x7FFFFFFFull)
+        fail(NBH_OVERFLOW_ERROR, "octree node count exceeds the walk's position range");
+    launch_walk_sentinel(s->stream, mixed(s) ? s->w32node.p : nullptr,
+                         mixed(s) ? nullptr : s->w64link.p, static_cast<int>(cap));
     s->mcap = cap;
 }
 

@@ -742,14 +742,10 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     }
     if (between)
         NBH_HIP(hipEventRecord(between, s));
-    const int M = o.n_nodes;
-    if (mixed)
-        link_sentinel_kernel<<<blocks_for(M, 256), 256, 0, s>>>(
-            w32pair_link, make_int4(0x7F800000, M, KIND_ERROR + 4, 0), M, 2);
-    else
-        link_sentinel_kernel<<<blocks_for(M, 256), 256, 0, s>>>(
-            w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M, 1);
-    NBH_HIP(hipGetLastError());
+    // The pipeline's octree comes from its own builder: every node lies on
+    // the root's tree and every position below the walk length is written
+    // here on each rebuild.  The error sentinel is written once, when the
+    // record buffers are allocated (launch_walk_sentinel).
     const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32pair_link,
                     w32center};
     for (int L = 0; L <= max_level; ++L) {
@@ -777,6 +773,21 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
 }
 
 size_t level_info_ints() { return kLevelInfoInts; }
+
+// Writes the error sentinel "error 4, skip to `cap`" into every link of
+// freshly allocated walk records.  Records are later overwritten only by
+// the layout passes, whose skip links always point forward, so a walk over
+// these buffers always ends.
+void launch_walk_sentinel(hipStream_t s, int4* w32node, int4* w64link, int cap)
+{
+    if (w32node)
+        link_sentinel_kernel<<<blocks_for(cap, 256), 256, 0, s>>>(
+            w32node + 1, make_int4(0x7F800000, cap, KIND_ERROR + 4, 0), cap, 2);
+    if (w64link)
+        link_sentinel_kernel<<<blocks_for(cap, 256), 256, 0, s>>>(
+            w64link, make_int4(cap, KIND_ERROR + 4, 0, 0), cap, 1);
+    NBH_HIP(hipGetLastError());
+}
 
 // The reference-layout per-node arrays of the pipeline's octree (mass and
 // centre of mass, cell centre and half size, child counts), filled from the
