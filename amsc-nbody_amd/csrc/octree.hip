@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void materialize_kernel(
         prefix[nd] = prefix_at<BITS>(key, l);
         o.parent[nd] = p;
         const int oct = octant_at<BITS>(key, l);
-        const int prev = atomicCAS(&o.children[p * 8 + oct], -1, nd);
+        const int prev = atomicCAS(&o.children[static_cast<size_t>(p) * 8 + oct], -1, nd);
         if (prev != -1 && prev != nd) {
             raise_error(err, 4);
             return;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void count_children_kernel(nbh_octree_view o,
         return;
     int c = 0;
     for (int oc = 0; oc < 8; ++oc) {
-        const int ch = o.children[node * 8 + oc];
+        const int ch = o.children[static_cast<size_t>(node) * 8 + oc];
         if (ch == -1)
             continue;
         if (ch <= 0 || ch >= o.n_nodes) {
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(nbh_octree_view o,
         }
         double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
         for (int oc = 0; oc < 8; ++oc) {
-            const int ch = o.children[p * 8 + oc];
+            const int ch = o.children[static_cast<size_t>(p) * 8 + oc];
             if (ch == -1)
                 continue;
             const double cm = o.mass[ch];

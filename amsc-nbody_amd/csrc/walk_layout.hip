@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void check_parents_kernel(nbh_octree_view o,
         return;
     int count = 0;
     for (int oc = 0; oc < 8; ++oc) {
-        const int c = o.children[node * 8 + oc];
+        const int c = o.children[static_cast<size_t>(node) * 8 + oc];
         if (c == -1)
             continue;
         if (c <= 0 || c >= o.n_nodes) {
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void generic_sizes_kernel(
         }
         int off = 1;
         for (int oc = 7; oc >= 0; --oc) {
-            const int c = o.children[p * 8 + oc];
+            const int c = o.children[static_cast<size_t>(p) * 8 + oc];
             if (c <= 0 || c >= o.n_nodes)
                 continue;
             local[c] = off;
@@ -227,13 +227,13 @@ __device__ __forceinline__ void write_record(
         const double oz = box[2] - side / 2.0;
         const double lim = 2147483647.0;
         if (kind == KIND_LEAF && count == 1) {
-            w32node[2 * pre] = rec32[first];
+            w32node[2 * static_cast<size_t>(pre)] = rec32[first];
         } else {
             const int qx = static_cast<int>(fmin(fmax(rint((cx - ox) * scale), 0.0), lim));
             const int qy = static_cast<int>(fmin(fmax(rint((cy - oy) * scale), 0.0), lim));
             const int qz = static_cast<int>(fmin(fmax(rint((cz - oz) * scale), 0.0), lim));
             const float gm = static_cast<float>(G * m * scale * scale);
-            w32node[2 * pre] = make_int4(qx, qy, qz, __float_as_int(gm));
+            w32node[2 * static_cast<size_t>(pre)] = make_int4(qx, qy, qz, __float_as_int(gm));
         }
         const double cq = crit * scale;
         float crit2 = static_cast<float>(cq * cq);
@@ -248,7 +248,7 @@ __device__ __forceinline__ void write_record(
             crit2 = -1.0f;
         else if (kind != KIND_INTERNAL)
             crit2 = __int_as_float(0x7F800000);
-        w32link[2 * pre] = make_int4(__float_as_int(crit2), skip,
+        w32link[2 * static_cast<size_t>(pre)] = make_int4(__float_as_int(crit2), skip,
                                  kind_out | (count << 8), first);
         if (need_center) {
             const double gx = GENERIC ? o.center_x[v] : geom.x;
@@ -328,8 +328,8 @@ __global__ __launch_bounds__(256) void level_up_kernel(
             }
             size[v] = 1;
         } else {
-            const int4 c0 = reinterpret_cast<const int4*>(o.children)[2 * v];
-            const int4 c1 = reinterpret_cast<const int4*>(o.children)[2 * v + 1];
+            const int4 c0 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v)];
+            const int4 c1 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v) + 1];
             const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
 #pragma unroll
             for (int oc = 0; oc < 8; ++oc) {
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(256) void octree_views_kernel(
     int nch = 0;
     if (o.particle_count[v] <= 0)
         for (int oc = 0; oc < 8; ++oc)
-            nch += o.children[8 * v + oc] != -1;
+            nch += o.children[8 * static_cast<size_t>(v) + oc] != -1;
     o.child_count[v] = nch;
     o.pending_children[v] = nch;
 }

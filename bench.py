@@ -184,6 +184,8 @@ def main():
                     help="extra opening angles for the accuracy table (Plummer only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-energy", action="store_true",
+                    help="skip the FP64 energy check (it builds a second, FP64 simulation)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -238,13 +240,13 @@ def main():
 
     s0 = mixed.state()
     p0 = momentum(s0)
-    e0 = energy_of(nb, s0, args)
+    e0 = None if args.no_energy else energy_of(nb, s0, args)
     for _ in range(args.warmup):
         mixed.step()
     secs, phases = time_steps(nb, torch, mixed, args.steps)
     s1 = mixed.state()
     p1 = momentum(s1)
-    e1 = energy_of(nb, s1, args)
+    e1 = None if args.no_energy else energy_of(nb, s1, args)
     ms_per_step = 1e3 * secs / args.steps
     value = n * args.steps / secs / 1e6
 
@@ -293,11 +295,11 @@ def main():
         "speedup_vs_fp64": None if fp64_rate is None else round(value / fp64_rate, 3),
         "accuracy": {k: float(f"{v:.4g}") for k, v in acc.items()},
         "momentum_drift": float(np.abs(p1 - p0).max()),
-        "energy": {"E_start": e0[0], "E_end": e1[0], "kinetic_start": e0[1],
-                   "potential_start": e0[2],
-                   "rel_drift": (e1[0] - e0[0]) / abs(e0[0]),
-                   "steps": args.warmup + args.steps,
-                   "fp64_rel_drift": energy64, "fp64_steps": args.fp64_steps},
+        "energy": None if e0 is None else {
+            "E_start": e0[0], "E_end": e1[0], "kinetic_start": e0[1],
+            "potential_start": e0[2], "rel_drift": (e1[0] - e0[0]) / abs(e0[0]),
+            "steps": args.warmup + args.steps,
+            "fp64_rel_drift": energy64, "fp64_steps": args.fp64_steps},
         "accuracy_by_theta": by_theta,
         "phase_ms": {k: round(v, 4) for k, v in phases.items()},
         "frame_capture": frames,

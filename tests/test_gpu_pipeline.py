@@ -260,3 +260,27 @@ def test_fp64_pipeline_bit_exact_plummer16k_ten_steps(nb, orc):
     for k, f in enumerate(("ax", "ay", "az")):
         assert np.array_equal(a[k], rs[f]), f
     compare_structure(sim.structure_host(), ref.structure())
+
+
+def test_mixed_pipeline_at_2_28_particles(nb):
+    # 2^28 Plummer particles build about 4e8 octree nodes: child-slot indices
+    # (8 per node) and walk-record indices pass 2^31 and must be 64-bit.
+    n = 1 << 28
+    print(f"N=2^28 mixed build + forces", flush=True)
+    p = nb.generate("plummer", n, 11)
+    sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
+                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    sim.initialize()
+    rng = np.random.default_rng(5)
+    targets = np.sort(rng.choice(n, 256, replace=False)).astype(np.uint32)
+    dev = [torch.from_numpy(p[k]).cuda() for k in ("m", "x", "y", "z")]
+    del p
+    exact = [t.cpu().numpy() for t in
+             nb.direct_sum(*dev, torch.from_numpy(targets.view(np.int32)).cuda(), 1e-3, 1.0)]
+    del dev
+    a = sim.accelerations()
+    rms, mx = rel_err([a[k][targets] for k in range(3)], exact)
+    print(f"N=2^28 mixed: BH vs direct rms={rms:.3e} max={mx:.3e}")
+    assert rms < 2e-3
+    sim.close()
+    torch.cuda.empty_cache()
