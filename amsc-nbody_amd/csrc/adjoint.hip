@@ -45,7 +45,7 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
                            double eps2, double G, bool contains,
                            const double* abx, const double* aby,
                            const double* abz, double* scratch, double* gx,
-                           double* gy, double* gz, int* err);
+                           double* gy, double* gz, double* gm, int* err);
 size_t force_vjp_scratch_doubles(size_t n, size_t wcount, size_t n_nodes);
 
 namespace {
@@ -109,9 +109,11 @@ void rebuild_here(nbh_sim* s)
     finish(s);
 }
 
-// xbar += (da/dx)^T abar at the simulation's current tree.  abar and xbar
-// are planar (x | y | z) arrays of n, original order.
-void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scratch)
+// xbar += (da/dx)^T abar and, when mbar is given, mbar += (da/dm)^T abar at
+// the simulation's current tree.  abar and xbar are planar (x | y | z)
+// arrays of n, mbar an array of n, original order.
+void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scratch,
+               double* mbar = nullptr)
 {
     const size_t n = s->n;
     const size_t M = static_cast<size_t>(s->nnodes);
@@ -126,7 +128,7 @@ void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scr
                           s->rec64.p, s->idx.p, static_cast<int>(n), eps * eps,
                           s->cfg.gravitational_constant, contains, abar,
                           abar + n, abar + 2 * n, scratch.p, xbar, xbar + n,
-                          xbar + 2 * n, s->errors + ERR_TRAVERSAL);
+                          xbar + 2 * n, mbar, s->errors + ERR_TRAVERSAL);
     s->mark(PH_KICK);
     s->mark(PH_COUNT);
     finish(s);
@@ -152,6 +154,7 @@ struct nbh_adjoint {
     nbh::DevBuf<double> ck;   // segments x 6n: x | y | z | vx | vy | vz
     nbh::DevBuf<double> seg;  // (every + 1) x 3n positions of one segment
     nbh::DevBuf<double> xb, vb, ab; // 3n each
+    nbh::DevBuf<double> mb;         // n
     nbh::DevBuf<double> scratch;
 
     ~nbh_adjoint() { delete sim; }
@@ -247,7 +250,7 @@ int nbh_sim_potential(nbh_sim* s, double* d_phi)
 
 int nbh_sim_force_vjp(nbh_sim* s, const double* d_abx, const double* d_aby,
                       const double* d_abz, double* d_gx, double* d_gy,
-                      double* d_gz)
+                      double* d_gz, double* d_gm)
 {
     return guard([&] {
         if (!s || !d_abx || !d_aby || !d_abz || !d_gx || !d_gy || !d_gz)
@@ -256,15 +259,17 @@ int nbh_sim_force_vjp(nbh_sim* s, const double* d_abx, const double* d_aby,
         const size_t n = s->n;
         DevBuf<double> ab, g, scratch;
         ab.alloc(3 * n);
-        g.alloc(3 * n);
+        g.alloc(4 * n);
         copy(s->stream, ab.p, d_abx, n);
         copy(s->stream, ab.p + n, d_aby, n);
         copy(s->stream, ab.p + 2 * n, d_abz, n);
-        launch_fill_f64(s->stream, g.p, 0.0, 3 * n);
-        force_vjp(s, ab.p, g.p, scratch);
+        launch_fill_f64(s->stream, g.p, 0.0, 4 * n);
+        force_vjp(s, ab.p, g.p, scratch, d_gm ? g.p + 3 * n : nullptr);
         copy(s->stream, d_gx, g.p, n);
         copy(s->stream, d_gy, g.p + n, n);
         copy(s->stream, d_gz, g.p + 2 * n, n);
+        if (d_gm)
+            copy(s->stream, d_gm, g.p + 3 * n, n);
         NBH_HIP(hipStreamSynchronize(s->stream));
     });
 }
@@ -296,6 +301,7 @@ int nbh_adjoint_create(nbh_adjoint** out, const nbh_sim_config* config,
             a->xb.alloc(3 * count);
             a->vb.alloc(3 * count);
             a->ab.alloc(3 * count);
+            a->mb.alloc(count);
         } catch (...) {
             delete a;
             throw;
@@ -351,7 +357,8 @@ int nbh_adjoint_backward(nbh_adjoint* a, const double* d_xbar, const double* d_y
                          const double* d_zbar, const double* d_vxbar,
                          const double* d_vybar, const double* d_vzbar,
                          double* d_x0bar, double* d_y0bar, double* d_z0bar,
-                         double* d_vx0bar, double* d_vy0bar, double* d_vz0bar)
+                         double* d_vx0bar, double* d_vy0bar, double* d_vz0bar,
+                         double* d_m0bar)
 {
     return guard([&] {
         if (!a || !d_xbar || !d_ybar || !d_zbar || !d_vxbar || !d_vybar ||
@@ -368,6 +375,10 @@ int nbh_adjoint_backward(nbh_adjoint* a, const double* d_xbar, const double* d_y
         double* xb = a->xb.p;
         double* vb = a->vb.p;
         double* ab = a->ab.p;
+        // every force evaluation a(x_j, m) adds its mass cotangent
+        double* mb = d_m0bar ? a->mb.p : nullptr;
+        if (mb)
+            launch_fill_f64(st, mb, 0.0, n);
         copy(st, xb, d_xbar, n);
         copy(st, xb + n, d_ybar, n);
         copy(st, xb + 2 * n, d_zbar, n);
@@ -392,7 +403,7 @@ int nbh_adjoint_backward(nbh_adjoint* a, const double* d_xbar, const double* d_y
                     set_positions(s, p, p + n, p + 2 * n);
                     rebuild_here(s);
                 }
-                force_vjp(s, ab, xb, a->scratch);
+                force_vjp(s, ab, xb, a->scratch, mb);
                 axpy3(st, dt, xb, vb, n);
             }
         }
@@ -401,13 +412,15 @@ int nbh_adjoint_backward(nbh_adjoint* a, const double* d_xbar, const double* d_y
         const double* p0 = a->checkpoint(0);
         set_positions(s, p0, p0 + n, p0 + 2 * n);
         rebuild_here(s);
-        force_vjp(s, ab, xb, a->scratch);
+        force_vjp(s, ab, xb, a->scratch, mb);
         copy(st, d_x0bar, xb, n);
         copy(st, d_y0bar, xb + n, n);
         copy(st, d_z0bar, xb + 2 * n, n);
         copy(st, d_vx0bar, vb, n);
         copy(st, d_vy0bar, vb + n, n);
         copy(st, d_vz0bar, vb + 2 * n, n);
+        if (mb)
+            copy(st, d_m0bar, mb, n);
         NBH_HIP(hipStreamSynchronize(st));
     });
 }

@@ -10,11 +10,16 @@
 //   phi_t -= G m r2^{-1/2}
 //   g      = (da_t/dd)^T abar_t = G m (r2^{-3/2} abar - 3 r2^{-5/2} d (d . abar))
 //   xbar_t -= g,  xbar_s += g
-// A monopole source is a centre of mass, COM_A = sum_k m_k x_k / M_A, so its
-// cotangent reaches particle k as (m_k / M_A) xbar_A: node cotangents are
-// pushed down the octree, accumulating xbar_A / M_A from the root, and each
-// leaf particle adds m_k times the sum.  Source cotangents are reduced over
-// the wavefront (every lane sees the same source) before one atomic add.
+//   mbar_s += (da_t/dm)^T abar_t = G r2^{-3/2} (d . abar)
+// A monopole source is a centre of mass, COM_A = sum_k m_k x_k / M_A with
+// M_A = sum_k m_k, so its position cotangent reaches particle k as
+// (m_k / M_A) xbar_A, and its mass cotangent as
+// mbar_A + (x_k - COM_A) . xbar_A / M_A.  Node cotangents are pushed down
+// the octree, accumulating xbar_A / M_A and mbar_A - COM_A . xbar_A / M_A
+// from the root; each leaf particle adds m_k times the first sum to its
+// position cotangent, and the second sum plus x_k . (first sum) to its mass
+// cotangent.  Source cotangents are reduced over the wavefront (every lane
+// sees the same source) before one atomic add per component.
 #include "walk_common.hpp"
 
 namespace nbh {
@@ -37,6 +42,8 @@ struct OpArgs {
     double* nbx;                   // node-source terms, by walk position
     double* nby;
     double* nbz;
+    double* pm;                    // particle-source mass terms, Morton order
+    double* nm;                    // node-source mass terms, by walk position
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -55,21 +62,25 @@ struct Lane {
     int code = 0;
 
     // One source for every lane of the wave (`use` false on lanes that do
-    // not interact).  Called from wave-uniform control flow.
+    // not interact).  Called from wave-uniform control flow.  The potential
+    // skips zero masses; the product keeps them for the mass cotangent (a
+    // is linear in every source mass), except at a singular separation,
+    // where a zero mass adds nothing either way.
     __device__ __forceinline__ void source(bool use, double sx, double sy,
                                            double sz, double sm, double G,
                                            double eps2, double* dx_out,
                                            double* dy_out, double* dz_out,
-                                           int index)
+                                           double* m_out, int index)
     {
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        if (use && sm != 0.0) {
+        double cx = 0.0, cy = 0.0, cz = 0.0, cm = 0.0;
+        if (use && (sm != 0.0 || OP == OP_VJP)) {
             const double dx = sx - tx;
             const double dy = sy - ty;
             const double dz = sz - tz;
             const double r2 = fma(dz, dz, fma(dx, dx, dy * dy)) + eps2;
             if (r2 <= 0.0) {
-                code = 6;
+                if (sm != 0.0)
+                    code = 6;
             } else {
                 const double inv = 1.0 / sqrt(r2);
                 const double gm = G * sm;
@@ -82,6 +93,7 @@ struct Lane {
                     cx = gm * fma(inv3, bx, -c5 * dx);
                     cy = gm * fma(inv3, by, -c5 * dy);
                     cz = gm * fma(inv3, bz, -c5 * dz);
+                    cm = G * dot * inv3;
                     gx -= cx;
                     gy -= cy;
                     gz -= cz;
@@ -89,14 +101,16 @@ struct Lane {
             }
         }
         if (OP == OP_VJP) {
-            if (__ballot(cx != 0.0 || cy != 0.0 || cz != 0.0) != 0ull) {
+            if (__ballot(cx != 0.0 || cy != 0.0 || cz != 0.0 || cm != 0.0) != 0ull) {
                 cx = wave_sum(cx);
                 cy = wave_sum(cy);
                 cz = wave_sum(cz);
+                cm = wave_sum(cm);
                 if ((threadIdx.x & 63) == 0) {
                     atomicAdd(dx_out + index, cx);
                     atomicAdd(dy_out + index, cy);
                     atomicAdd(dz_out + index, cz);
+                    atomicAdd(m_out + index, cm);
                 }
             }
         }
@@ -170,19 +184,19 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
                 }
             }
             L.source(accept, nd.x, nd.y, nd.z, nd.w, G, eps2, a.nbx, a.nby,
-                     a.nbz, i);
+                     a.nbz, a.nm, i);
         } else if (kind == KIND_LEAF) {
             const int first = lk.z;
             const int count = lk.w;
             if (count == 1) {
                 const double4 s = wnode[i];
                 L.source(active && first != t, s.x, s.y, s.z, s.w, G, eps2,
-                         a.pbx, a.pby, a.pbz, first);
+                         a.pbx, a.pby, a.pbz, a.pm, first);
             } else {
                 for (int k = first; k < first + count; ++k) {
                     const double4 s = rec[k];
                     L.source(active && k != t, s.x, s.y, s.z, s.w, G, eps2,
-                             a.pbx, a.pby, a.pbz, k);
+                             a.pbx, a.pby, a.pbz, a.pm, k);
                 }
             }
             if (active)
@@ -210,58 +224,70 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
     }
 }
 
-// Top-down over one level: s_v = s_parent + nbar(pre_v) / M_v, then every
-// particle k of a leaf v adds m_k s_v to its source cotangent.
+// Top-down over one level: u_v = u_parent + nbar(pre_v) / M_v and
+// w_v = w_parent + mbar(pre_v) - COM_v . nbar(pre_v) / M_v; then every
+// particle k of a leaf v adds m_k u_v to its position cotangent and
+// w_v + x_k . u_v to its mass cotangent.
 __global__ __launch_bounds__(256) void push_down_kernel(
     nbh_octree_view o, int L, const int* __restrict__ pre_of, int wcount,
     const double* __restrict__ nbx, const double* __restrict__ nby,
-    const double* __restrict__ nbz, double* __restrict__ sx,
-    double* __restrict__ sy, double* __restrict__ sz,
-    const double4* __restrict__ rec, double* __restrict__ pbx,
-    double* __restrict__ pby, double* __restrict__ pbz)
+    const double* __restrict__ nbz, const double* __restrict__ nm,
+    double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz,
+    double* __restrict__ sw, const double4* __restrict__ rec,
+    double* __restrict__ pbx, double* __restrict__ pby, double* __restrict__ pbz,
+    double* __restrict__ pm)
 {
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
     for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
          v < o.n_nodes; v += stride) {
         if (o.level[v] != L)
             continue;
-        double ux = 0.0, uy = 0.0, uz = 0.0;
+        double ux = 0.0, uy = 0.0, uz = 0.0, uw = 0.0;
         if (L > 0) {
             const int p = o.parent[v];
             ux = sx[p];
             uy = sy[p];
             uz = sz[p];
+            uw = sw[p];
         }
         const int pre = pre_of[v];
         const double m = o.mass[v];
         if (m > 0.0 && pre >= 0 && pre < wcount) {
-            ux += nbx[pre] / m;
-            uy += nby[pre] / m;
-            uz += nbz[pre] / m;
+            const double bx = nbx[pre] / m;
+            const double by = nby[pre] / m;
+            const double bz = nbz[pre] / m;
+            ux += bx;
+            uy += by;
+            uz += bz;
+            uw += nm[pre] - fma(o.com_z[v], bz, fma(o.com_x[v], bx, o.com_y[v] * by));
         }
         sx[v] = ux;
         sy[v] = uy;
         sz[v] = uz;
+        sw[v] = uw;
         const int pc = o.particle_count[v];
         if (pc > 0) {
             const int f = o.first_particle[v];
             for (int k = f; k < f + pc; ++k) {
-                const double mk = rec[k].w;
-                pbx[k] = fma(mk, ux, pbx[k]);
-                pby[k] = fma(mk, uy, pby[k]);
-                pbz[k] = fma(mk, uz, pbz[k]);
+                const double4 r = rec[k];
+                pbx[k] = fma(r.w, ux, pbx[k]);
+                pby[k] = fma(r.w, uy, pby[k]);
+                pbz[k] = fma(r.w, uz, pbz[k]);
+                pm[k] += uw + fma(r.z, uz, fma(r.x, ux, r.y * uy));
             }
         }
     }
 }
 
-// xbar (original order) += target terms + source terms of Morton slot t.
+// xbar (original order) += target terms + source terms of Morton slot t;
+// mbar (original order, optional) += mass terms.
 __global__ __launch_bounds__(256) void combine_kernel(
     const uint32_t* __restrict__ idx, int n, const double* __restrict__ tbx,
     const double* __restrict__ tby, const double* __restrict__ tbz,
     const double* __restrict__ pbx, const double* __restrict__ pby,
-    const double* __restrict__ pbz, double* __restrict__ gx,
-    double* __restrict__ gy, double* __restrict__ gz, int* err)
+    const double* __restrict__ pbz, const double* __restrict__ pm,
+    double* __restrict__ gx, double* __restrict__ gy, double* __restrict__ gz,
+    double* __restrict__ gm, int* err)
 {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= n)
@@ -274,6 +300,8 @@ __global__ __launch_bounds__(256) void combine_kernel(
     gx[o] += tbx[t] + pbx[t];
     gy[o] += tby[t] + pby[t];
     gz[o] += tbz[t] + pbz[t];
+    if (gm)
+        gm[o] += pm[t];
 }
 
 // Per-block partial sums of 1/2 m v^2 and 1/2 m phi in a fixed order.
@@ -361,17 +389,20 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
                            double eps2, double G, bool contains,
                            const double* abx, const double* aby,
                            const double* abz, double* scratch, double* gx,
-                           double* gy, double* gz, int* err)
+                           double* gy, double* gz, double* gm, int* err)
 {
-    // scratch: tb (3n) | pb (3n) | nb (3 wcount) | s (3 n_nodes)
+    // scratch: tb (3n) | pb (3n) | pm (n) | nb (3 wcount) | nm (wcount) |
+    //          u (3 n_nodes) | w (n_nodes)
     const size_t N = static_cast<size_t>(n);
     const size_t W = static_cast<size_t>(wcount_host);
     const size_t M = static_cast<size_t>(o.n_nodes);
     double* tb = scratch;
     double* pb = tb + 3 * N;
-    double* nbar = pb + 3 * N;
-    double* sv = nbar + 3 * W;
-    launch_fill_f64(s, pb, 0.0, 3 * N + 3 * W);
+    double* pm = pb + 3 * N;
+    double* nbar = pm + N;
+    double* nm = nbar + 3 * W;
+    double* sv = nm + W;
+    launch_fill_f64(s, pb, 0.0, 4 * N + 4 * W);
     OpArgs a{};
     a.abx = abx;
     a.aby = aby;
@@ -385,6 +416,8 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
     a.nbx = nbar;
     a.nby = nbar + W;
     a.nbz = nbar + 2 * W;
+    a.pm = pm;
+    a.nm = nm;
     launch_op<OP_VJP>(s, wnode, wcrit, wlink, wcenter, d_wcount, rec, idx, n,
                       eps2, G, contains, a, err);
     unsigned blocks = blocks_for(M, 256);
@@ -392,19 +425,19 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
         blocks = 2048;
     for (int L = 0; L <= max_level; ++L) {
         push_down_kernel<<<blocks, 256, 0, s>>>(
-            o, L, pre_of, wcount_host, a.nbx, a.nby, a.nbz, sv, sv + M,
-            sv + 2 * M, rec, a.pbx, a.pby, a.pbz);
+            o, L, pre_of, wcount_host, a.nbx, a.nby, a.nbz, a.nm, sv, sv + M,
+            sv + 2 * M, sv + 3 * M, rec, a.pbx, a.pby, a.pbz, a.pm);
         NBH_HIP(hipGetLastError());
     }
     combine_kernel<<<blocks_for(N, 256), 256, 0, s>>>(idx, n, a.tbx, a.tby,
                                                      a.tbz, a.pbx, a.pby, a.pbz,
-                                                     gx, gy, gz, err);
+                                                     a.pm, gx, gy, gz, gm, err);
     NBH_HIP(hipGetLastError());
 }
 
 size_t force_vjp_scratch_doubles(size_t n, size_t wcount, size_t n_nodes)
 {
-    return 6 * n + 3 * wcount + 3 * n_nodes;
+    return 7 * n + 4 * wcount + 4 * n_nodes;
 }
 
 } // namespace nbh

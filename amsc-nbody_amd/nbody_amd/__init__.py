@@ -186,11 +186,11 @@ _sig("nbh_particle_file_info", _i, C.c_char_p, C.POINTER(_sz), C.POINTER(_i))
 _sig("nbh_particle_file_read", _i, C.c_char_p, _sz, _p, _p, _p, _p, _p, _p, _p, _p,
      C.POINTER(_sz))
 _sig("nbh_sim_potential", _i, _p, _p)
-_sig("nbh_sim_force_vjp", _i, _p, _p, _p, _p, _p, _p, _p)
+_sig("nbh_sim_force_vjp", _i, _p, _p, _p, _p, _p, _p, _p, _p)
 _sig("nbh_adjoint_create", _i, C.POINTER(_p), C.POINTER(SimConfig), _sz, _i, _i, _p)
 _sig("nbh_adjoint_destroy", _i, _p)
 _sig("nbh_adjoint_forward", _i, _p, *([_p] * 13))
-_sig("nbh_adjoint_backward", _i, _p, *([_p] * 12))
+_sig("nbh_adjoint_backward", _i, _p, *([_p] * 13))
 
 
 class FrameHeader(C.Structure):
@@ -1034,13 +1034,15 @@ class Simulation:
         check(lib.nbh_sim_potential(self.handle, ptr(phi)))
         return phi
 
-    def force_vjp(self, abx, aby, abz):
-        """(da/dx)^T abar at the current tree; device float64 tensors in,
-        device tensors out (FP64 simulations only)."""
+    def force_vjp(self, abx, aby, abz, with_mass=False):
+        """(da/dx)^T abar at the current tree, and (da/dm)^T abar as a fourth
+        output with with_mass; device float64 tensors in, device tensors out
+        (FP64 simulations only)."""
         torch = _torch()
         ab = [t.to(device=device(), dtype=torch.float64).contiguous() for t in (abx, aby, abz)]
-        g = [empty(self.count, torch.float64) for _ in range(3)]
-        check(lib.nbh_sim_force_vjp(self.handle, *[ptr(t) for t in ab], *[ptr(t) for t in g]))
+        g = [empty(self.count, torch.float64) for _ in range(4 if with_mass else 3)]
+        check(lib.nbh_sim_force_vjp(self.handle, *[ptr(t) for t in ab], *[ptr(t) for t in g[:3]],
+                                    ptr(g[3]) if with_mass else None))
         return g
 
 
@@ -1080,20 +1082,23 @@ class Adjoint:
                                       *[ptr(t) for t in outs]))
         return outs
 
-    def backward(self, xb, yb, zb, vxb, vyb, vzb):
+    def backward(self, xb, yb, zb, vxb, vyb, vzb, with_mass=False):
+        """Cotangents of (x0, y0, z0, vx0, vy0, vz0), and of the masses as a
+        seventh output with with_mass."""
         torch = _torch()
         ins = [t.to(device=device(), dtype=torch.float64).contiguous()
                for t in (xb, yb, zb, vxb, vyb, vzb)]
-        outs = [empty(self.count, torch.float64) for _ in range(6)]
+        outs = [empty(self.count, torch.float64) for _ in range(7 if with_mass else 6)]
         check(lib.nbh_adjoint_backward(self.handle, *[ptr(t) for t in ins],
-                                       *[ptr(t) for t in outs]))
+                                       *[ptr(t) for t in outs[:6]],
+                                       ptr(outs[6]) if with_mass else None))
         return outs
 
 
 def leapfrog_function():
     """torch.autograd.Function over Adjoint: (m, pos[3,N], vel[3,N]) ->
-    (pos_K, vel_K); gradients reach pos and vel (masses are constants).
-    For checking the adjoint against finite differences."""
+    (pos_K, vel_K); gradients reach m, pos and vel.  For checking the
+    adjoint against finite differences."""
     torch = _torch()
 
     class LeapfrogBH(torch.autograd.Function):
@@ -1110,8 +1115,11 @@ def leapfrog_function():
                 gpos = torch_.zeros(3, ctx.adjoint.count, dtype=torch_.float64, device=device())
             if gvel is None:
                 gvel = torch_.zeros(3, ctx.adjoint.count, dtype=torch_.float64, device=device())
-            g = ctx.adjoint.backward(gpos[0], gpos[1], gpos[2], gvel[0], gvel[1], gvel[2])
-            return None, None, torch_.stack(g[:3]), torch_.stack(g[3:])
+            with_mass = ctx.needs_input_grad[1]
+            g = ctx.adjoint.backward(gpos[0], gpos[1], gpos[2], gvel[0], gvel[1], gvel[2],
+                                     with_mass=with_mass)
+            return (None, g[6] if with_mass else None, torch_.stack(g[:3]),
+                    torch_.stack(g[3:6]))
 
     return LeapfrogBH
 

@@ -316,19 +316,22 @@ int nbh_sim_energy(nbh_sim* sim, double* kinetic, double* potential);
 int nbh_sim_potential(nbh_sim* sim, double* d_phi);
 
 /* Force vector-Jacobian product at the current tree of an FP64 simulation:
- * g = (da/dx)^T abar, the tree (interaction lists) held fixed.  All arrays
- * are device arrays of count in original order.  Pairs with
- * nbh_sim_accelerations as the forward. */
+ * g = (da/dx)^T abar and, when d_gm != NULL, gm = (da/dm)^T abar, the tree
+ * (interaction lists) held fixed.  A source node's mass and centre of mass
+ * are differentiated through to its particles.  All arrays are device arrays
+ * of count in original order.  Pairs with nbh_sim_accelerations as the
+ * forward. */
 int nbh_sim_force_vjp(nbh_sim* sim, const double* d_abx, const double* d_aby,
                       const double* d_abz, double* d_gx, double* d_gy,
-                      double* d_gz);
+                      double* d_gz, double* d_gm);
 
 /* Reverse mode of `steps` kick-drift-kick steps (FP64 pipeline, tree fixed
  * within each force evaluation).  Forward takes the initial state (device
  * arrays, original order), stores (x, v) every checkpoint_every steps and
  * returns the final state; backward takes the cotangents of the final
- * positions and velocities and returns those of the initial ones, recomputing
- * each segment from its checkpoint.  Masses are not differentiated. */
+ * positions and velocities and returns those of the initial positions,
+ * velocities and, when d_m0bar != NULL, masses, recomputing each segment
+ * from its checkpoint. */
 typedef struct nbh_adjoint nbh_adjoint;
 int nbh_adjoint_create(nbh_adjoint** out, const nbh_sim_config* config,
                        size_t count, int steps, int checkpoint_every,
@@ -345,7 +348,7 @@ int nbh_adjoint_backward(nbh_adjoint* adjoint, const double* d_xbar,
                          const double* d_vxbar, const double* d_vybar,
                          const double* d_vzbar, double* d_x0bar,
                          double* d_y0bar, double* d_z0bar, double* d_vx0bar,
-                         double* d_vy0bar, double* d_vz0bar);
+                         double* d_vy0bar, double* d_vz0bar, double* d_m0bar);
 
 /* ------------------------------------------------------------ workloads */
 /* Seeded synthetic particle sets (host arrays of length count).

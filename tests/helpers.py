@@ -245,6 +245,17 @@ def direct_vjp_numpy(m, pos, abar, eps, G=1.0):
     return g.sum(1) - g.sum(2)                     # sources gain, targets lose
 
 
+def direct_mass_vjp_numpy(m, pos, abar, eps, G=1.0):
+    """(da/dm)^T abar of the softened direct sum.  a_i = sum_j G m_j d_ij
+    r_ij^-3 is linear in each source mass, so mbar_j = sum_{i != j} G
+    abar_i . d_ij r_ij^-3 (d_ij = x_j - x_i).  Written from the closed form."""
+    d = pos[:, None, :] - pos[:, :, None]          # (3, i, j) = x_j - x_i
+    r2 = (d * d).sum(0) + eps * eps
+    np.fill_diagonal(r2, np.inf)
+    dot = (d * abar[:, :, None]).sum(0)            # d_ij . abar_i
+    return G * (dot * r2 ** -1.5).sum(0)
+
+
 def leapfrog_numpy(m, pos, vel, dt, steps, eps, G=1.0):
     """KDK with the direct sum; returns the positions x_0..x_K and the final
     velocities."""
@@ -262,17 +273,23 @@ def leapfrog_numpy(m, pos, vel, dt, steps, eps, G=1.0):
     return xs, v
 
 
-def leapfrog_adjoint_numpy(m, pos, vel, dt, steps, eps, xbar, vbar, G=1.0):
-    """Reverse sweep of leapfrog_numpy: cotangents of (x_0, v_0) from those
-    of (x_K, v_K)."""
+def leapfrog_adjoint_numpy(m, pos, vel, dt, steps, eps, xbar, vbar, G=1.0,
+                           with_mass=False):
+    """Reverse sweep of leapfrog_numpy: cotangents of (x_0, v_0), and of the
+    masses with with_mass, from those of (x_K, v_K).  Every force
+    evaluation a(x_j) adds (da/dm)^T of its cotangent to mbar."""
     xs, _ = leapfrog_numpy(m, pos, vel, dt, steps, eps, G)
     xb, vb = xbar.copy(), vbar.copy()
+    mb = np.zeros_like(m)
     for j in range(steps, 0, -1):
         ab = (0.5 * dt if j == steps else dt) * vb
         xb = xb + direct_vjp_numpy(m, xs[j], ab, eps, G)
+        mb = mb + direct_mass_vjp_numpy(m, xs[j], ab, eps, G)
         vb = vb + dt * xb
-    xb = xb + direct_vjp_numpy(m, xs[0], 0.5 * dt * vb, eps, G)
-    return xb, vb
+    ab = 0.5 * dt * vb
+    xb = xb + direct_vjp_numpy(m, xs[0], ab, eps, G)
+    mb = mb + direct_mass_vjp_numpy(m, xs[0], ab, eps, G)
+    return (xb, vb, mb) if with_mass else (xb, vb)
 
 
 def truncated_plummer_energy(rmax=10.0, G=1.0):

@@ -31,6 +31,45 @@ def test_direct_vjp_matches_finite_differences():
     assert np.linalg.norm(g - fd) / np.linalg.norm(fd) < 1e-8
 
 
+def test_direct_mass_vjp_matches_finite_differences():
+    m, pos, _, rng = _system(14, 3)
+    eps = 0.05
+    ab = rng.normal(size=pos.shape)
+    g = H.direct_mass_vjp_numpy(m, pos, ab, eps)
+    h = 1e-7
+    fd = np.zeros_like(m)
+    for j in range(len(m)):
+        m1, m2 = m.copy(), m.copy()
+        m1[j] += h
+        m2[j] -= h
+        a1 = np.stack(H.direct_numpy(m1, *pos, eps))
+        a2 = np.stack(H.direct_numpy(m2, *pos, eps))
+        fd[j] = ((a1 - a2) * ab).sum() / (2 * h)
+    # a is linear in m: the central difference is exact up to rounding
+    assert np.linalg.norm(g - fd) / np.linalg.norm(fd) < 1e-7
+
+
+def test_direct_adjoint_mass_gradient_matches_finite_differences():
+    m, pos, vel, rng = _system(10, 4)
+    eps, dt, K = 0.05, 1e-2, 5
+    xbar, vbar = rng.normal(size=pos.shape), rng.normal(size=pos.shape)
+    _, _, mb = H.leapfrog_adjoint_numpy(m, pos, vel, dt, K, eps, xbar, vbar,
+                                        with_mass=True)
+
+    def loss(mm):
+        xs, vk = H.leapfrog_numpy(mm, pos, vel, dt, K, eps)
+        return (xs[-1] * xbar).sum() + (vk * vbar).sum()
+
+    h = 1e-7
+    fd = np.zeros_like(m)
+    for j in range(len(m)):
+        m1, m2 = m.copy(), m.copy()
+        m1[j] += h
+        m2[j] -= h
+        fd[j] = (loss(m1) - loss(m2)) / (2 * h)
+    assert np.linalg.norm(mb - fd) / np.linalg.norm(fd) < 1e-6
+
+
 def test_direct_adjoint_matches_finite_differences():
     m, pos, vel, rng = _system(12, 1)
     eps, dt, K = 0.05, 1e-2, 6

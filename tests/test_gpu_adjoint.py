@@ -1,4 +1,5 @@
-"""Energy, force vector-Jacobian product and the K-step adjoint (GPU).
+"""Energy, force vector-Jacobian product and the K-step adjoint (GPU), for
+positions, velocities and masses.
 
 Pinned by: the numpy direct sum at theta = 0 (potential, VJP, adjoint: the
 Barnes-Hut walk with theta = 0 opens every node, so it is the direct sum);
@@ -134,6 +135,47 @@ def test_force_vjp_matches_finite_differences(nb):
     compare_probes(gg, fd, ok, "force VJP vs FD on 64 coordinates")
 
 
+def test_force_vjp_mass_theta0_equals_direct(nb):
+    p = nb.generate("uniform", 400, 12)
+    p["m"] = np.random.default_rng(7).uniform(0.5, 1.5, 400) / 400
+    pos, _ = planar(p)
+    ab = np.random.default_rng(8).normal(size=pos.shape)
+    s = sim_of(nb, p, 0.0, 0.02)
+    out = s.force_vjp(*[torch.from_numpy(a) for a in ab], with_mass=True)
+    g = np.stack([t.cpu().numpy() for t in out[:3]])
+    gm = out[3].cpu().numpy()
+    assert rel(g, H.direct_vjp_numpy(p["m"], pos, ab, 0.02)) < 1e-11
+    assert rel(gm, H.direct_mass_vjp_numpy(p["m"], pos, ab, 0.02)) < 1e-11
+
+
+def test_force_vjp_mass_matches_finite_differences(nb):
+    # At theta = 0.5 most sources are node monopoles: the mass cotangent goes
+    # through each node's mass and centre of mass to its particles.
+    n = 3000
+    p = nb.generate("plummer", n, 13)
+    p["m"] = p["m"] * np.random.default_rng(9).uniform(0.5, 1.5, n)
+    eps = 0.01
+    pos, _ = planar(p)
+    rng = np.random.default_rng(10)
+    ab = rng.normal(size=pos.shape)
+    s = sim_of(nb, p, 0.5, eps)
+    gm = s.force_vjp(*[torch.from_numpy(a) for a in ab], with_mass=True)[3].cpu().numpy()
+    a0 = np.stack(s.accelerations())
+    fd, ok, ad = [], [], []
+    for i in rng.choice(n, 64, False):
+        def f(delta, i=int(i)):
+            q = dict(p)
+            q["m"] = p["m"].copy()
+            q["m"][i] += delta
+            a = np.stack(sim_of(nb, q, 0.5, eps).accelerations())
+            return ((a - a0) * ab).sum()
+        d, good = central_fd(f, 1e-4 * p["m"][i])
+        fd.append(d)
+        ok.append(good)
+        ad.append(gm[i])
+    compare_probes(ad, fd, ok, "force mass VJP vs FD on 64 masses")
+
+
 # --------------------------------------------------------------- adjoint
 def test_adjoint_theta0_equals_direct_adjoint(nb):
     n, K, dt, eps = 64, 6, 1e-2, 0.05
@@ -150,6 +192,23 @@ def test_adjoint_theta0_equals_direct_adjoint(nb):
     xb, vb = H.leapfrog_adjoint_numpy(p["m"], pos, vel, dt, K, eps, xbar, vbar)
     assert rel(np.stack([t.cpu().numpy() for t in g[:3]]), xb) < 1e-10
     assert rel(np.stack([t.cpu().numpy() for t in g[3:]]), vb) < 1e-10
+
+
+def test_adjoint_mass_theta0_equals_direct_adjoint(nb):
+    n, K, dt, eps = 64, 6, 1e-2, 0.05
+    p = nb.generate("blob", n, 14)
+    p["m"] = np.random.default_rng(11).uniform(0.5, 1.5, n) / n
+    pos, vel = planar(p)
+    rng = np.random.default_rng(12)
+    xbar, vbar = rng.normal(size=pos.shape), rng.normal(size=pos.shape)
+    adj = nb.Adjoint(n, dt, 0.0, eps, K, 4, gravitational_constant=1.0)
+    adj.forward(*[torch.from_numpy(a) for a in (p["m"], *pos, *vel)])
+    g = adj.backward(*[torch.from_numpy(a) for a in (*xbar, *vbar)], with_mass=True)
+    xb, vb, mb = H.leapfrog_adjoint_numpy(p["m"], pos, vel, dt, K, eps, xbar, vbar,
+                                          with_mass=True)
+    assert rel(np.stack([t.cpu().numpy() for t in g[:3]]), xb) < 1e-10
+    assert rel(np.stack([t.cpu().numpy() for t in g[3:6]]), vb) < 1e-10
+    assert rel(g[6].cpu().numpy(), mb) < 1e-10
 
 
 def _disk_problem(nb, n=2048, K=20, every=5):
@@ -196,6 +255,44 @@ def test_adjoint_matches_finite_differences(nb):
     compare_probes(ad, fd, ok, f"adjoint (disk N={n}, K=20, checkpoints every 5) vs FD")
 
 
+def test_adjoint_mass_gradient_matches_finite_differences(nb):
+    # the differentiable workload's check: gradient of the final state with
+    # respect to the masses, central differences on 64 of them
+    p, pos, vel, adj = _disk_problem(nb)
+    n = pos.shape[1]
+    rng = np.random.default_rng(13)
+    w, u = rng.normal(size=pos.shape), rng.normal(size=pos.shape)
+    state = [torch.from_numpy(a) for a in (*pos, *vel)]
+
+    def final(m):
+        out = adj.forward(torch.from_numpy(m), *state)
+        return np.stack([t.cpu().numpy() for t in out])
+
+    base = final(p["m"])
+
+    def loss(m):
+        d = final(m) - base
+        return (d[:3] * w).sum() + (d[3:] * u).sum()
+
+    final(p["m"])
+    gm = adj.backward(*[torch.from_numpy(a) for a in (*w, *u)], with_mass=True)[6]
+    gm = gm.cpu().numpy()
+    fd, ok, ad = [], [], []
+    for i in rng.choice(n, 64, False):
+        def f(delta, i=int(i)):
+            m = p["m"].copy()
+            m[i] += delta
+            return loss(m)
+        # a mass step moves the centres of mass of the particle's nodes by
+        # the same relative amount: 1e-5 keeps opening decisions stable
+        # over the 20 steps while the rounding of the loss stays ~1e-6
+        d, good = central_fd(f, 1e-5 * p["m"][i])
+        fd.append(d)
+        ok.append(good)
+        ad.append(gm[i])
+    compare_probes(ad, fd, ok, f"adjoint mass gradient (disk N={n}, K=20) vs FD")
+
+
 def test_adjoint_independent_of_checkpoint_interval(nb):
     n, K = 1024, 9
     p = nb.generate("disk", n, 10)
@@ -206,8 +303,8 @@ def test_adjoint_independent_of_checkpoint_interval(nb):
     for every in (1, 4, K):
         adj = nb.Adjoint(n, 1e-3, 0.5, 0.01, K, every, gravitational_constant=1.0)
         adj.forward(*[torch.from_numpy(a) for a in (p["m"], *pos, *vel)])
-        g = adj.backward(*[torch.from_numpy(a) for a in (*w, *u)])
-        res.append(np.stack([t.cpu().numpy() for t in g]))
+        g = adj.backward(*[torch.from_numpy(a) for a in (*w, *u)], with_mass=True)
+        res.append(np.concatenate([t.cpu().numpy() for t in g]))
         adj.close()
     # node cotangents are summed with atomics: equal up to summation order
     assert rel(res[1], res[0]) < 1e-12
@@ -238,11 +335,14 @@ def test_autograd_function_matches_adjoint(nb):
     x0 = torch.tensor(pos, device=dev, requires_grad=True)
     v0 = torch.tensor(vel, device=dev, requires_grad=True)
     w = torch.randn(3, n, dtype=torch.float64, device=dev, generator=None)
-    xk, vk = fn.apply(adj, torch.from_numpy(p["m"]).to(dev), x0, v0)
+    m = torch.tensor(p["m"], device=dev, requires_grad=True)
+    xk, vk = fn.apply(adj, m, x0, v0)
     (xk * w).sum().backward()
-    ref = adj.backward(*w, *torch.zeros(3, n, dtype=torch.float64, device=dev))
+    ref = adj.backward(*w, *torch.zeros(3, n, dtype=torch.float64, device=dev),
+                       with_mass=True)
     assert rel(x0.grad.cpu().numpy(), np.stack([t.cpu().numpy() for t in ref[:3]])) < 1e-12
-    assert rel(v0.grad.cpu().numpy(), np.stack([t.cpu().numpy() for t in ref[3:]])) < 1e-12
+    assert rel(v0.grad.cpu().numpy(), np.stack([t.cpu().numpy() for t in ref[3:6]])) < 1e-12
+    assert rel(m.grad.cpu().numpy(), ref[6].cpu().numpy()) < 1e-12
 
 
 # ------------------------------------------------------------- two bodies
