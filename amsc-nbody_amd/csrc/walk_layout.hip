@@ -398,9 +398,21 @@ __global__ __launch_bounds__(256) void level_down_kernel(
          k < end; k += stride) {
         const int v = order[k];
         int pre = 0;
-        if (L == 0)
-            *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
-        else
+        if (L == 0) {
+            const int wc = size[0] < o.n_nodes ? size[0] : o.n_nodes;
+            *wcount = wc;
+            // Terminal record at the walk length ("error 4, skip to the
+            // end"): a walk reaches position wc only by finishing, and the
+            // traversal's internal-node loop stops at any record of another
+            // kind, so it needs no separate bound test.  The buffers hold
+            // more than n_nodes records, and position wc may still hold a
+            // record of an earlier, longer walk.
+            if (MIXED)
+                w.w32link[2 * static_cast<size_t>(wc)] =
+                    make_int4(0x7F800000, wc, KIND_ERROR + 4, 0);
+            else
+                w.w64link[wc] = make_int4(wc, KIND_ERROR + 4, 0, 0);
+        } else
             pre = pre_of[o.parent[v]] + local[v];
         pre_of[v] = pre;
         if (pre < 0 || pre >= o.n_nodes)
@@ -744,8 +756,10 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
         NBH_HIP(hipEventRecord(between, s));
     // The pipeline's octree comes from its own builder: every node lies on
     // the root's tree and every position below the walk length is written
-    // here on each rebuild.  The error sentinel is written once, when the
-    // record buffers are allocated (launch_walk_sentinel).
+    // here on each rebuild, as is the terminal record at the walk length.
+    // The error sentinel of the other positions is written once, when the
+    // record buffers are allocated (launch_walk_sentinel); the buffers hold
+    // at least n_nodes + 1024 records.
     const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32pair_link,
                     w32center};
     for (int L = 0; L <= max_level; ++L) {

@@ -74,20 +74,278 @@ __device__ __forceinline__ void separation(const int4& s, const int4& t,
     d2 = fmaf(dz, dz, fmaf(dx, dx, dy * dy));
 }
 
-// Softened FP32 monopole; `f` is forced to zero where `use` is false.
+// Softened FP32 monopole added to (x, y, z); `f` is forced to zero where
+// `use` is false.
 __device__ __forceinline__ void interact(float dx, float dy, float dz, float d2,
                                          float gm, float eps2, bool use,
-                                         Acc& a)
+                                         float& x, float& y, float& z)
 {
     const float r2 = d2 + eps2;
     const float inv = __builtin_amdgcn_rsqf(r2);
     float f = gm * (inv * inv * inv);
     f = use ? f : 0.0f;
-    a.px = fmaf(f, dx, a.px);
-    a.py = fmaf(f, dy, a.py);
-    a.pz = fmaf(f, dz, a.pz);
+    x = fmaf(f, dx, x);
+    y = fmaf(f, dy, y);
+    z = fmaf(f, dz, z);
 }
 
+constexpr int kIdle = 0x7FFFFFFF;
+
+// One packet's walk: 64 Morton-consecutive targets, one per lane.  A lane's
+// `next` is the first position it still has to see.  At position i a lane
+// is active when next <= i.  Accepting (or finishing a leaf) sets
+// next = skip; opening leaves next <= i, so the lane is active again at
+// i + 1.  Lanes waiting for a later position always wait for skip[i] or
+// beyond (subtrees nest), hence the packet moves to i + 1 when some lane
+// opened and to skip[i] otherwise.
+struct Walker {
+    int4 tq;       // target (fixed point)
+    int t;         // sorted index of the target
+    bool valid;    // t < n
+    int next;
+    int code;      // traversal error of this lane, 0 if none
+    int i;         // current position (wave-uniform)
+    Acc a;
+    unsigned long long npp, npn, visits;
+
+    __device__ __forceinline__ void init(const int4* __restrict__ rec, int packet,
+                                         int lane, int n)
+    {
+        t = packet * 64 + lane;
+        valid = t < n;
+        tq = valid ? rec[t] : make_int4(0, 0, 0, 0);
+        next = valid ? 0 : kIdle;
+        code = 0;
+        i = 0;
+        npp = npn = visits = 0;
+    }
+};
+
+// Lane masks of the two comparisons of a visit, written by the compare
+// instructions straight into a scalar register pair (exec is full here).
+__device__ __forceinline__ unsigned long long mask_le(int next, int i)
+{
+    unsigned long long m;
+    asm("v_cmp_ge_i32_e64 %0, %1, %2" : "=s"(m) : "s"(i), "v"(next));
+    return m;
+}
+
+__device__ __forceinline__ unsigned long long mask_gt(float d2, float crit2)
+{
+    unsigned long long m;
+    asm("v_cmp_lt_f32_e64 %0, %1, %2" : "=s"(m) : "s"(crit2), "v"(d2));
+    return m;
+}
+
+// Per lane: m's bit of the lane ? b : a.
+__device__ __forceinline__ float select_f(unsigned long long m, float a, float b)
+{
+    float r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(m));
+    return r;
+}
+
+__device__ __forceinline__ int select_i(unsigned long long m, int a, int b)
+{
+    int r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(m));
+    return r;
+}
+
+__device__ __forceinline__ bool lane_bit(unsigned long long m)
+{
+    return ((m >> (threadIdx.x & 63)) & 1ull) != 0ull;
+}
+
+// The straight-line part of a visit, for records of every kind: the MAC
+// and, when some lane accepts, the monopole and the lanes' next position.
+// Records of kinds other than internal carry crit^2 = +inf and no lane
+// accepts them here.  Returns the active and accepting lane masks; they are
+// ballots of the two comparisons themselves, so they stay the comparisons'
+// lane masks instead of a bool that is materialised per lane and compared
+// again.
+template <bool CONTAINS, bool COUNT, bool BRANCH = true>
+__device__ __forceinline__ void visit_mac(Walker& w, const WalkRec& r,
+                                          const int4* __restrict__ wcenter,
+                                          float eps2, unsigned long long& actm,
+                                          unsigned long long& accm)
+{
+    const int4 lk = r.link;
+    const int4 nd = r.node;
+    float dx, dy, dz, d2;
+    separation(nd, w.tq, dx, dy, dz, d2);
+    actm = mask_le(w.next, w.i);
+    accm = actm & mask_gt(d2, __int_as_float(lk.x));
+    if (CONTAINS && accm != 0ull) {
+        bool acc = lane_bit(accm);
+        if (acc) {
+            const int4 c = wcenter[w.i];
+            acc = !(abs(w.tq.x - c.x) <= c.w && abs(w.tq.y - c.y) <= c.w &&
+                    abs(w.tq.z - c.z) <= c.w);
+        }
+        accm = __ballot(acc);
+    }
+    if (!BRANCH || accm != 0ull) {
+        const float r2 = d2 + eps2;
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float f = select_f(accm, 0.0f, __int_as_float(nd.w) * (inv * inv * inv));
+        w.a.px = fmaf(f, dx, w.a.px);
+        w.a.py = fmaf(f, dy, w.a.py);
+        w.a.pz = fmaf(f, dz, w.a.pz);
+        if (COUNT && lane_bit(accm) && __int_as_float(nd.w) != 0.0f) {
+            if ((lk.z >> 8) == 1) {
+                if (lk.w != w.t)
+                    ++w.npp;
+            } else {
+                ++w.npn;
+            }
+        }
+        w.next = select_i(accm, w.next, lk.y);
+    }
+}
+
+typedef int int8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ WalkRec as_record(const int8v& v)
+{
+    WalkRec r;
+    r.node = make_int4(v[0], v[1], v[2], v[3]);
+    r.link = make_int4(v[4], v[5], v[6], v[7]);
+    return r;
+}
+
+// Loads the records of both packets with one wait for the two: the second
+// load issues before the first has returned.  The wait is inside the same
+// statement, so no output register is written after the statement ends.
+__device__ __forceinline__ void load_pair(const WalkRec* pa, const WalkRec* pb,
+                                          WalkRec& ra, WalkRec& rb)
+{
+    int8v a, b;
+    asm volatile("s_load_dwordx8 %0, %2, 0x0\n\t"
+                 "s_load_dwordx8 %1, %3, 0x0\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&s"(a), "=&s"(b)
+                 : "s"(pa), "s"(pb)
+                 : "memory");
+    ra = as_record(a);
+    rb = as_record(b);
+}
+
+// Next position after an internal-node visit: the active lanes that opened
+// the node are still at or before i (skip > i, so an accepting lane has
+// left).
+__device__ __forceinline__ void advance_internal(Walker& w, const WalkRec& r,
+                                                 unsigned long long actm,
+                                                 unsigned long long accm)
+{
+    w.i = (accm != actm) ? w.i + 1 : r.link.y;
+}
+
+// A record of another kind below the walk length, after visit_mac: a
+// multi-particle leaf (or a single particle without softening), a
+// zero-mass node to skip, or an error record.
+template <bool SOFT, bool COUNT>
+__device__ __forceinline__ void visit_rare(Walker& w, const WalkRec& r,
+                                        const int4* __restrict__ rec,
+                                        float eps2)
+{
+    const int4 lk = r.link;
+    const int skip = lk.y;
+    const int kind = lk.z & 0xFF;
+    const bool active = w.next <= w.i;
+    if (kind == KIND_LEAF) {
+        const int count = lk.z >> 8;
+        const int first = lk.w;
+        for (int k = first; k < first + count; ++k) {
+            const int4 s = rec[k];
+            float sx, sy, sz, s2;
+            separation(s, w.tq, sx, sy, sz, s2);
+            const float gm = __int_as_float(s.w);
+            bool use = active && k != w.t && gm != 0.0f;
+            if (!SOFT && use && s2 == 0.0f) {
+                w.code = 6;
+                use = false;
+            }
+            interact(sx, sy, sz, s2, gm, eps2, use, w.a.px, w.a.py, w.a.pz);
+            if (COUNT && use)
+                ++w.npp;
+        }
+        w.next = active ? skip : w.next;
+    } else if (kind == KIND_SKIP) {
+        w.next = active ? skip : w.next;
+    } else if (active) {
+        w.code = (kind == KIND_CHILDLESS) ? 5 : kind - KIND_ERROR;
+    }
+    if (w.code != 0)
+        w.next = kIdle;
+    w.i = (__ballot(w.next <= w.i) != 0ull) ? w.i + 1 : skip;
+}
+
+// Walks one packet from its current position to the end.  Position wcount
+// holds a record of a non-internal kind, which ends the internal-node loop.
+template <bool CONTAINS, bool SOFT, bool COUNT>
+__device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__ wrec,
+                                          const int4* __restrict__ wcenter,
+                                          const int4* __restrict__ rec,
+                                          int wcount, float eps2, int& fold)
+{
+    for (;;) {
+        WalkRec r;
+        unsigned long long actm, accm;
+        for (;;) {
+            r = wrec[w.i];
+            visit_mac<CONTAINS, COUNT>(w, r, wcenter, eps2, actm, accm);
+            if ((r.link.z & 0xFF) != KIND_INTERNAL)
+                break;
+            if (COUNT)
+                ++w.visits;
+            if (++fold == kFold) {
+                w.a.fold();
+                fold = 0;
+            }
+            advance_internal(w, r, actm, accm);
+        }
+        if (w.i >= wcount)
+            break;
+        if (COUNT)
+            ++w.visits;
+        visit_rare<SOFT, COUNT>(w, r, rec, eps2);
+        if (++fold == kFold) {
+            w.a.fold();
+            fold = 0;
+        }
+    }
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ idx,
+                                       int n, double* __restrict__ ax_out,
+                                       double* __restrict__ ay_out,
+                                       double* __restrict__ az_out, int* err)
+{
+    w.a.fold();
+    if (w.code != 0) {
+        raise_error(err, w.code);
+        return;
+    }
+    if (COUNT || !w.valid)
+        return;
+    const uint32_t o = idx[w.t];
+    if (o >= static_cast<uint32_t>(n)) {
+        raise_error(err, 3);
+        return;
+    }
+    ax_out[o] = w.a.kx.value();
+    ay_out[o] = w.a.ky.value();
+    az_out[o] = w.a.kz.value();
+}
+
+// Each wavefront walks two adjacent packets in lock-step: one iteration
+// loads both packets' records, then does both visits, so the two record
+// loads (mostly served by L2: the walk's working set exceeds the scalar
+// cache) are in flight together and each visit's VALU work overlaps the
+// other packet's load.  A packet at a rare record kind or at its end leaves
+// the lock-step loop; the other one then continues alone.
 template <bool CONTAINS, bool SOFT, bool COUNT>
 __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const WalkRec* __restrict__ wrec,
@@ -97,103 +355,77 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     double* __restrict__ ay_out, double* __restrict__ az_out, int* err,
     unsigned long long* __restrict__ counters)
 {
-    constexpr int kIdle = 0x7FFFFFFF;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int packet = packet_of_block(nblocks_padded) * 4 + wave;
-    const int t = packet * 64 + lane;
-    if (packet * 64 >= n)
+    const int pair = packet_of_block(nblocks_padded) * 4 + wave;
+    const int pa = 2 * pair;
+    const int pb = pa + 1;
+    if (pa * 64 >= n)
         return;
-    const bool valid = t < n;
+    const bool has_b = pb * 64 < n;
     const int wcount = *d_wcount;
-    int4 tq = make_int4(0, 0, 0, 0);
-    if (valid)
-        tq = rec[t];
-    Acc a;
-    unsigned long long npp = 0, npn = 0, visits = 0;
-    int next = valid ? 0 : kIdle;
-    int code = 0;
+    Walker A, B;
+    A.init(rec, pa, lane, n);
+    B.init(rec, pb, lane, n);
     int fold = 0;
-    int i = 0;
-    // A lane's `next` is the first position it still has to see.  At
-    // position i a lane is active when next <= i.  Accepting (or finishing a
-    // leaf) sets next = skip; opening leaves next <= i, so the lane is active
-    // again at i + 1.  Lanes waiting for a later position always wait for
-    // skip[i] or beyond (subtrees nest), hence the wave moves to i + 1 when
-    // some lane opened and to skip[i] otherwise.
-    while (i < wcount) {
-        const WalkRec r = wrec[i];
-        const int4 lk = r.link;
-        const int4 nd = r.node;
-        const int skip = lk.y;
-        const int kind = lk.z & 0xFF;
-        const bool active = next <= i;
-        if (COUNT)
-            ++visits;
-        // Straight-line MAC for every record: records of other kinds carry
-        // crit^2 = +inf and are never accepted here.
-        float dx, dy, dz, d2;
-        separation(nd, tq, dx, dy, dz, d2);
-        bool acc = active && d2 > __int_as_float(lk.x);
-        if (CONTAINS && acc) {
-            const int4 c = wcenter[i];
-            const bool inside = abs(tq.x - c.x) <= c.w &&
-                                abs(tq.y - c.y) <= c.w &&
-                                abs(tq.z - c.z) <= c.w;
-            acc = !inside;
-        }
-        if (__ballot(acc) != 0ull) {
-            interact(dx, dy, dz, d2, __int_as_float(nd.w), eps2, acc, a);
-            if (COUNT && acc && __int_as_float(nd.w) != 0.0f) {
-                if ((lk.z >> 8) == 1) {
-                    if (lk.w != t)
-                        ++npp;
-                } else {
-                    ++npn;
-                }
+    bool run_a = true, run_b = has_b;
+    while (run_a && run_b) {
+        // kFold lock-step iterations between folds; the loop body has no
+        // branch besides its exit, so both record loads issue together.
+        WalkRec ra, rb;
+        unsigned long long act_a, acc_a, act_b, acc_b;
+        bool rare;
+        int left = kFold;
+        for (;;) {
+            load_pair(wrec + A.i, wrec + B.i, ra, rb);
+            visit_mac<CONTAINS, COUNT, false>(A, ra, wcenter, eps2, act_a, acc_a);
+            visit_mac<CONTAINS, COUNT, false>(B, rb, wcenter, eps2, act_b, acc_b);
+            rare = ((ra.link.z | rb.link.z) & 0xFF) != KIND_INTERNAL;
+            if (rare || --left == 0)
+                break;
+            if (COUNT) {
+                ++A.visits;
+                ++B.visits;
             }
+            advance_internal(A, ra, act_a, acc_a);
+            advance_internal(B, rb, act_b, acc_b);
         }
-        next = acc ? skip : next;
-        if (kind != KIND_INTERNAL) {
-            if (kind == KIND_LEAF) {
-                // multi-particle leaf, or a single particle without softening
-                const int count = lk.z >> 8;
-                const int first = lk.w;
-                for (int k = first; k < first + count; ++k) {
-                    const int4 s = rec[k];
-                    float sx, sy, sz, s2;
-                    separation(s, tq, sx, sy, sz, s2);
-                    const float gm = __int_as_float(s.w);
-                    bool use = active && k != t && gm != 0.0f;
-                    if (!SOFT && use && s2 == 0.0f) {
-                        code = 6;
-                        use = false;
-                    }
-                    interact(sx, sy, sz, s2, gm, eps2, use, a);
-                    if (COUNT && use)
-                        ++npp;
-                }
-                next = active ? skip : next;
-            } else if (kind == KIND_SKIP) {
-                next = active ? skip : next;
-            } else if (active) {
-                code = (kind == KIND_CHILDLESS) ? 5 : kind - KIND_ERROR;
-            }
-            if (code != 0)
-                next = kIdle;
+        A.a.fold();
+        B.a.fold();
+        // Finish both visits of the last iteration.
+        if ((ra.link.z & 0xFF) == KIND_INTERNAL) {
+            if (COUNT)
+                ++A.visits;
+            advance_internal(A, ra, act_a, acc_a);
+        } else if (A.i >= wcount) {
+            run_a = false;
+        } else {
+            if (COUNT)
+                ++A.visits;
+            visit_rare<SOFT, COUNT>(A, ra, rec, eps2);
         }
-        if (++fold == kFold) {
-            a.fold();
-            fold = 0;
+        if ((rb.link.z & 0xFF) == KIND_INTERNAL) {
+            if (COUNT)
+                ++B.visits;
+            advance_internal(B, rb, act_b, acc_b);
+        } else if (B.i >= wcount) {
+            run_b = false;
+        } else {
+            if (COUNT)
+                ++B.visits;
+            visit_rare<SOFT, COUNT>(B, rb, rec, eps2);
         }
-        i = (__ballot(next <= i) != 0ull) ? i + 1 : skip;
+        (void)rare;
     }
-    a.fold();
-    if (code != 0) {
-        raise_error(err, code);
-        return;
-    }
+    if (run_a)
+        walk_rest<CONTAINS, SOFT, COUNT>(A, wrec, wcenter, rec, wcount, eps2, fold);
+    if (run_b)
+        walk_rest<CONTAINS, SOFT, COUNT>(B, wrec, wcenter, rec, wcount, eps2, fold);
+    finish<COUNT>(A, idx, n, ax_out, ay_out, az_out, err);
+    if (has_b)
+        finish<COUNT>(B, idx, n, ax_out, ay_out, az_out, err);
     if (COUNT) {
+        unsigned long long npp = A.npp + B.npp, npn = A.npn + B.npn;
         for (int o = 32; o > 0; o >>= 1) {
             npp += __shfl_xor(npp, o);
             npn += __shfl_xor(npn, o);
@@ -201,25 +433,22 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
         if (lane == 0) {
             atomicAdd(&counters[0], npp);
             atomicAdd(&counters[1], npn);
-            atomicAdd(&counters[2], visits);
+            atomicAdd(&counters[2], A.visits + B.visits);
         }
-        return;
-    }
-    if (valid) {
-        const uint32_t o = idx[t];
-        if (o >= static_cast<uint32_t>(n)) {
-            raise_error(err, 3);
-            return;
-        }
-        ax_out[o] = a.kx.value();
-        ay_out[o] = a.ky.value();
-        az_out[o] = a.kz.value();
     }
 }
 
 } // namespace
 
-int walk_blocks_padded(int n);
+// Blocks of four wavefronts, two packets each, padded to a multiple of the
+// eight XCDs.
+static int walk_pair_blocks_padded(int n)
+{
+    const int packets = (n + 63) / 64;
+    const int pairs = (packets + 1) / 2;
+    const int blocks = (pairs + 3) / 4;
+    return (blocks + 7) / 8 * 8;
+}
 
 void launch_walk_mixed(hipStream_t s, const int4* wpairs,
                        const int4* wcenter, const int* d_wcount,
@@ -228,7 +457,7 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs,
                        double* ay, double* az, int* err,
                        unsigned long long* counters)
 {
-    const int nb = walk_blocks_padded(n);
+    const int nb = walk_pair_blocks_padded(n);
     const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
 #define NBH_ARGS                                                                 \
     wrec, wcenter, d_wcount, rec, idx, n, nb, eps2, ax, ay, az, err,       \
