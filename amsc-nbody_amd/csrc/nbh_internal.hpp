@@ -251,11 +251,13 @@ void launch_walk_fp64(hipStream_t s, const double4* wnode, const double* wcrit,
                       bool contains, double* ax, double* ay, double* az,
                       int* err, unsigned long long* counters);
 // wpairs: interleaved {node, link} int4 pairs, 32 bytes per position.
+// queue: kWalkQueueInts ints of work-queue counters (cleared by the launch).
+constexpr int kWalkQueueInts = 256;
 void launch_walk_mixed(hipStream_t s, const int4* wpairs,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx, int n,
                        float eps2, bool contains, bool soft, double* ax,
-                       double* ay, double* az, int* err,
+                       double* ay, double* az, int* queue, int* err,
                        unsigned long long* counters);
 void launch_direct(hipStream_t s, const double* m, const double* x,
                    const double* y, const double* z, size_t n,

@@ -13,7 +13,7 @@ template <int BITS>
 void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          bool mixed, const double4* rec64, const int4* rec32,
                          double theta, const double* box, double G,
-                         bool need_center, bool soft, int* size, int* local,
+                         bool need_center, double eps, int* size, int* local,
                          double* crit, int* pre, int* order, int* level_info,
                          double4* sum4, const int* poffset, const int* pcount,
                          int n_groups, double4* w64node, double* w64crit,
@@ -126,6 +126,7 @@ struct nbh_sim {
     nbh::DevBuf<double> partials; // bbox partials
     nbh::DevBuf<int> scalars;     // [1] ngroups [2] nnodes [3] wcount ([0] unused)
     nbh::DevBuf<unsigned long long> counters;
+    nbh::DevBuf<int> walkq;       // mixed traversal work-queue counters
 
     // octree (capacity grows)
     size_t mcap = 0;

@@ -83,6 +83,7 @@ void alloc_state(nbh_sim* s)
     s->partials.alloc(bbox_partial_doubles());
     s->scalars.alloc(16);
     s->counters.alloc(4);
+    s->walkq.alloc(kWalkQueueInts);
     NBH_HIP(hipMemsetAsync(s->scalars.p, 0, 16 * sizeof(int), s->stream));
     NBH_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->rb), sizeof(Readback),
                           hipHostMallocDefault));
@@ -209,7 +210,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
     launch_level_passes<BITS>(
         st, o, Layout<BITS>::bits, mixed(s), s->rec64.p, s->rec32.p,
         s->cfg.theta, s->box.p, s->cfg.gravitational_constant, contains,
-        s->cfg.softening > 0.0, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->sum4.p, s->poffset.p, s->pcount.p, s->ngroups,
+        s->cfg.softening, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->sum4.p, s->poffset.p, s->pcount.p, s->ngroups,
         s->w64node.p, s->w64crit.p,
         s->w64link.p, s->w64center.p, s->w32node.p, nullptr,
         s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr);
@@ -230,8 +231,8 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters)
         launch_walk_mixed(st, s->w32node.p, s->w32center.p,
                           s->scalars.p + 3, s->rec32.p, s->idx.p, ni,
                           static_cast<float>(eq * eq), contains, eps > 0.0,
-                          a[0].p, a[1].p, a[2].p, s->errors + ERR_TRAVERSAL,
-                          counters);
+                          a[0].p, a[1].p, a[2].p, s->walkq.p,
+                          s->errors + ERR_TRAVERSAL, counters);
     } else {
         launch_walk_fp64(st, s->w64node.p, s->w64crit.p, s->w64link.p,
                          s->w64center.p, s->scalars.p + 3, s->rec64.p, s->idx.p,

@@ -142,7 +142,7 @@ __device__ __forceinline__ void write_record(
     const nbh_octree_view& o, int v, int pre, const int* __restrict__ size,
     const double* __restrict__ crit_in, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, double G, bool need_center, bool soft,
+    const double* __restrict__ box, double G, bool need_center, double eps,
     const WalkOut& w, const double4* __restrict__ sum4, double4 geom)
 {
     double4* w64node = w.w64node;
@@ -235,13 +235,18 @@ __device__ __forceinline__ void write_record(
             const float gm = static_cast<float>(G * m * scale * scale);
             w32node[2 * static_cast<size_t>(pre)] = make_int4(qx, qy, qz, __float_as_int(gm));
         }
+        // The traversal compares the softened squared distance d^2 + eps^2
+        // (its interaction's r^2) with crit^2 + eps^2, both in fixed-point
+        // units.
         const double cq = crit * scale;
-        float crit2 = static_cast<float>(cq * cq);
+        const double eq = eps * scale;
+        float crit2 = static_cast<float>(fma(cq, cq, eq * eq));
         // With positive softening a single-particle leaf and a zero-mass
         // node are "always accept" monopoles (crit^2 = -1): the target's own
         // particle and a zero mass both add exactly zero.  Records of every
         // other non-internal kind carry crit^2 = +inf, so that the
         // traversal's MAC never accepts them.
+        const bool soft = eps > 0.0;
         const bool fast = soft && ((kind == KIND_LEAF && count == 1) || kind == KIND_SKIP);
         const int kind_out = fast ? KIND_INTERNAL : kind;
         if (fast)
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
     if (v == 0)
         *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
     write_record<GENERIC, MIXED>(o, v, pre, size, crit_in, theta, rec64, rec32,
-                                 box, G, need_center, false, w, nullptr,
+                                 box, G, need_center, 0.0, w, nullptr,
                                  make_double4(0.0, 0.0, 0.0, 0.0));
 }
 
@@ -386,7 +391,7 @@ __global__ __launch_bounds__(256) void level_down_kernel(
     const int* __restrict__ local, int* __restrict__ pre_of,
     const double* __restrict__ crit, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, double G, bool need_center, bool soft,
+    const double* __restrict__ box, double G, bool need_center, double eps,
     WalkOut w, int* __restrict__ wcount, const int* __restrict__ order,
     const int* __restrict__ start, const double4* __restrict__ sum4)
 {
@@ -424,7 +429,7 @@ __global__ __launch_bounds__(256) void level_down_kernel(
                                 geom.x, geom.y, geom.z, geom.w, code);
         }
         write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32,
-                                   box, G, need_center, soft, w, sum4, geom);
+                                   box, G, need_center, eps, w, sum4, geom);
     }
 }
 
@@ -618,7 +623,7 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
     const int* __restrict__ local, int* __restrict__ pre_of,
     const double* __restrict__ crit, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, double G, bool need_center, bool soft,
+    const double* __restrict__ box, double G, bool need_center, double eps,
     WalkOut w, const double4* __restrict__ sum4)
 {
     using Key = typename Layout<BITS>::Key;
@@ -643,7 +648,7 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
                             geom.x, geom.y, geom.z, geom.w, code);
     }
     write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32, box, G,
-                               need_center, soft, w, sum4, geom);
+                               need_center, eps, w, sum4, geom);
 }
 
 // Every link starts as "error 4, skip to the end", so a position that the
@@ -710,7 +715,7 @@ template <int BITS>
 void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          bool mixed, const double4* rec64, const int4* rec32,
                          double theta, const double* box, double G,
-                         bool need_center, bool soft, int* size, int* local, double* crit,
+                         bool need_center, double eps, int* size, int* local, double* crit,
                          int* pre, int* order, int* level_info, double4* sum4,
                          const int* poffset, const int* pcount, int n_groups,
                          double4* w64node, double* w64crit,
@@ -766,11 +771,11 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
         if (mixed)
             level_down_kernel<BITS, true><<<blocks, 256, 0, s>>>(
                 o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
-                need_center, soft, w, wcount, order, start, sum4);
+                need_center, eps, w, wcount, order, start, sum4);
         else
             level_down_kernel<BITS, false><<<blocks, 256, 0, s>>>(
                 o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
-                need_center, soft, w, wcount, order, start, sum4);
+                need_center, eps, w, wcount, order, start, sum4);
         NBH_HIP(hipGetLastError());
     }
     // Leaves last: a leaf that collapsed its single-child parent overwrites
@@ -778,11 +783,11 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     if (mixed)
         leaf_down_kernel<BITS, true><<<gblocks, 256, 0, s>>>(
             o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
-            box, G, need_center, soft, w, sum4);
+            box, G, need_center, eps, w, sum4);
     else
         leaf_down_kernel<BITS, false><<<gblocks, 256, 0, s>>>(
             o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
-            box, G, need_center, soft, w, sum4);
+            box, G, need_center, eps, w, sum4);
     NBH_HIP(hipGetLastError());
 }
 
@@ -821,7 +826,7 @@ template void launch_octree_views<21>(hipStream_t, const nbh_octree_view&,
 #define NBH_LEVEL_INST(B)                                                            \
     template void launch_level_passes<B>(hipStream_t, const nbh_octree_view&, int, bool, \
                                          const double4*, const int4*, double,           \
-                                         const double*, double, bool, bool, int*, int*, \
+                                         const double*, double, bool, double, int*, int*, \
                                          double*, int*, int*, int*, double4*,           \
                                          const int*, const int*, int, double4*,          \
                                          double*, int4*,                                 \

@@ -22,11 +22,15 @@
 // compensated total carries the rounding of at most kFold terms per fold.
 #include "walk_common.hpp"
 
+#include <algorithm>
+
 namespace nbh {
 
 namespace {
 
 constexpr int kFold = 32;
+// ints between the work-queue counters of two ranges (one per 128 bytes)
+constexpr int kQueueStride = kWalkQueueInts / 8;
 
 // One position of the mixed walk: the node (fixed-point COM, G m) and its
 // link (crit^2, skip, kind | count << 8, first particle).
@@ -74,6 +78,18 @@ __device__ __forceinline__ void separation(const int4& s, const int4& t,
     d2 = fmaf(dz, dz, fmaf(dx, dx, dy * dy));
 }
 
+// Separation and softened squared distance d^2 + eps^2, the r^2 of the
+// interaction, which the MAC compares with the record's crit^2 + eps^2.
+__device__ __forceinline__ void separation_soft(const int4& s, const int4& t,
+                                                float eps2, float& dx, float& dy,
+                                                float& dz, float& r2)
+{
+    dx = static_cast<float>(s.x - t.x);
+    dy = static_cast<float>(s.y - t.y);
+    dz = static_cast<float>(s.z - t.z);
+    r2 = fmaf(dz, dz, fmaf(dx, dx, fmaf(dy, dy, eps2)));
+}
+
 // Softened FP32 monopole added to (x, y, z); `f` is forced to zero where
 // `use` is false.
 __device__ __forceinline__ void interact(float dx, float dy, float dz, float d2,
@@ -108,8 +124,10 @@ struct Walker {
     Acc a;
     unsigned long long npp, npn, visits;
 
-    __device__ __forceinline__ void init(const int4* __restrict__ rec, int packet,
-                                         int lane, int n)
+    // Starts the walk of `packet`; the interaction and visit counts go on
+    // accumulating over the packets a slot walks.
+    __device__ __forceinline__ void start(const int4* __restrict__ rec, int packet,
+                                          int lane, int n)
     {
         t = packet * 64 + lane;
         valid = t < n;
@@ -117,7 +135,7 @@ struct Walker {
         next = valid ? 0 : kIdle;
         code = 0;
         i = 0;
-        npp = npn = visits = 0;
+        a = Acc();
     }
 };
 
@@ -164,18 +182,28 @@ __device__ __forceinline__ bool lane_bit(unsigned long long m)
 // ballots of the two comparisons themselves, so they stay the comparisons'
 // lane masks instead of a bool that is materialised per lane and compared
 // again.
+// The record's skip link read through the vector memory path: the value
+// arrives in a vector register, where the lanes' next-position select can
+// take it directly.  (The select can read only one scalar operand, its lane
+// mask.)  A wavefront-scope relaxed load is a plain vector load.
+__device__ __forceinline__ int skip_vgpr(const WalkRec* r)
+{
+    return __hip_atomic_load(&r->link.y, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 template <bool CONTAINS, bool COUNT, bool BRANCH = true>
-__device__ __forceinline__ void visit_mac(Walker& w, const WalkRec& r,
+__device__ __forceinline__ void visit_mac(Walker& w, const WalkRec& r, int skipv,
                                           const int4* __restrict__ wcenter,
                                           float eps2, unsigned long long& actm,
                                           unsigned long long& accm)
 {
     const int4 lk = r.link;
     const int4 nd = r.node;
-    float dx, dy, dz, d2;
-    separation(nd, w.tq, dx, dy, dz, d2);
+    float dx, dy, dz, r2;
+    separation_soft(nd, w.tq, eps2, dx, dy, dz, r2);
     actm = mask_le(w.next, w.i);
-    accm = actm & mask_gt(d2, __int_as_float(lk.x));
+    accm = actm & mask_gt(r2, __int_as_float(lk.x));
     if (CONTAINS && accm != 0ull) {
         bool acc = lane_bit(accm);
         if (acc) {
@@ -186,7 +214,6 @@ __device__ __forceinline__ void visit_mac(Walker& w, const WalkRec& r,
         accm = __ballot(acc);
     }
     if (!BRANCH || accm != 0ull) {
-        const float r2 = d2 + eps2;
         const float inv = __builtin_amdgcn_rsqf(r2);
         const float f = select_f(accm, 0.0f, __int_as_float(nd.w) * (inv * inv * inv));
         w.a.px = fmaf(f, dx, w.a.px);
@@ -200,7 +227,7 @@ __device__ __forceinline__ void visit_mac(Walker& w, const WalkRec& r,
                 ++w.npn;
             }
         }
-        w.next = select_i(accm, w.next, lk.y);
+        w.next = select_i(accm, w.next, skipv);
     }
 }
 
@@ -294,7 +321,7 @@ __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__
         unsigned long long actm, accm;
         for (;;) {
             r = wrec[w.i];
-            visit_mac<CONTAINS, COUNT>(w, r, wcenter, eps2, actm, accm);
+            visit_mac<CONTAINS, COUNT>(w, r, r.link.y, wcenter, eps2, actm, accm);
             if ((r.link.z & 0xFF) != KIND_INTERNAL)
                 break;
             if (COUNT)
@@ -340,47 +367,87 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
     az_out[o] = w.a.kz.value();
 }
 
-// Each wavefront walks two adjacent packets in lock-step: one iteration
-// loads both packets' records, then does both visits, so the two record
-// loads (mostly served by L2: the walk's working set exceeds the scalar
-// cache) are in flight together and each visit's VALU work overlaps the
-// other packet's load.  A packet at a rare record kind or at its end leaves
-// the lock-step loop; the other one then continues alone.
+// The work queue: packets are dealt from eight ranges, one eighth of the
+// Morton order each, through one counter per range.  A wavefront takes
+// packets from the range of its own XCD, so each XCD's L2 serves one region
+// of space, and from the other ranges once its own is empty.
+struct Queue {
+    int* counters; // counter of range r at counters[r * kQueueStride]
+    int packets;
+    int per;       // packets per range
+    int home;      // this wavefront's XCD
+    int tried;     // ranges found empty so far, in order home, home + 1, ...
+
+    __device__ __forceinline__ int take()
+    {
+        while (tried < 8) {
+            const int r = (home + tried) & 7;
+            const int lo = r * per;
+            const int hi = min(lo + per, packets);
+            int k = 0;
+            if ((threadIdx.x & 63) == 0)
+                k = atomicAdd(&counters[r * kQueueStride], 1);
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (lo + k < hi)
+                return lo + k;
+            ++tried;
+        }
+        return -1;
+    }
+};
+
+__device__ __forceinline__ int xcd_id()
+{
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return static_cast<int>(v & 7u);
+}
+
+// Persistent wavefronts, each with two packet slots walked in lock-step:
+// one iteration loads both slots' records, then does both visits, so the
+// two record loads (mostly served by L2: the walk's working set exceeds the
+// scalar cache) are in flight together and each visit's VALU work overlaps
+// the other slot's load.  A slot whose packet ends writes its results and
+// takes the next packet from the queue, so the slots stay paired until the
+// queue is empty; the remaining slot then finishes alone.  Every wavefront
+// leaves once the queue is empty and its slots are done.
 template <bool CONTAINS, bool SOFT, bool COUNT>
 __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const WalkRec* __restrict__ wrec,
     const int4* __restrict__ wcenter, const int* __restrict__ d_wcount,
     const int4* __restrict__ rec, const uint32_t* __restrict__ idx, int n,
-    int nblocks_padded, float eps2, double* __restrict__ ax_out,
+    int* __restrict__ queue, float eps2, double* __restrict__ ax_out,
     double* __restrict__ ay_out, double* __restrict__ az_out, int* err,
     unsigned long long* __restrict__ counters)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int pair = packet_of_block(nblocks_padded) * 4 + wave;
-    const int pa = 2 * pair;
-    const int pb = pa + 1;
-    if (pa * 64 >= n)
+    const int packets = (n + 63) / 64;
+    Queue q{queue, packets, (packets + 7) / 8, xcd_id(), 0};
+    const int pa = q.take();
+    if (pa < 0)
         return;
-    const bool has_b = pb * 64 < n;
     const int wcount = *d_wcount;
     Walker A, B;
-    A.init(rec, pa, lane, n);
-    B.init(rec, pb, lane, n);
-    int fold = 0;
-    bool run_a = true, run_b = has_b;
+    A.npp = A.npn = A.visits = 0;
+    B.npp = B.npn = B.visits = 0;
+    A.start(rec, pa, lane, n);
+    const int pb = q.take();
+    bool run_a = true, run_b = pb >= 0;
+    if (run_b)
+        B.start(rec, pb, lane, n);
     while (run_a && run_b) {
         // kFold lock-step iterations between folds; the loop body has no
         // branch besides its exit, so both record loads issue together.
         WalkRec ra, rb;
         unsigned long long act_a, acc_a, act_b, acc_b;
-        bool rare;
         int left = kFold;
         for (;;) {
+            const int sa = skip_vgpr(wrec + A.i);
+            const int sb = skip_vgpr(wrec + B.i);
             load_pair(wrec + A.i, wrec + B.i, ra, rb);
-            visit_mac<CONTAINS, COUNT, false>(A, ra, wcenter, eps2, act_a, acc_a);
-            visit_mac<CONTAINS, COUNT, false>(B, rb, wcenter, eps2, act_b, acc_b);
-            rare = ((ra.link.z | rb.link.z) & 0xFF) != KIND_INTERNAL;
+            visit_mac<CONTAINS, COUNT, false>(A, ra, sa, wcenter, eps2, act_a, acc_a);
+            visit_mac<CONTAINS, COUNT, false>(B, rb, sb, wcenter, eps2, act_b, acc_b);
+            const bool rare = ((ra.link.z | rb.link.z) & 0xFF) != KIND_INTERNAL;
             if (rare || --left == 0)
                 break;
             if (COUNT) {
@@ -392,38 +459,50 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
         }
         A.a.fold();
         B.a.fold();
-        // Finish both visits of the last iteration.
+        // Finish both visits of the last iteration; a slot at the end of
+        // its walk moves on to the next packet.
         if ((ra.link.z & 0xFF) == KIND_INTERNAL) {
             if (COUNT)
                 ++A.visits;
             advance_internal(A, ra, act_a, acc_a);
-        } else if (A.i >= wcount) {
-            run_a = false;
-        } else {
+        } else if (A.i < wcount) {
             if (COUNT)
                 ++A.visits;
             visit_rare<SOFT, COUNT>(A, ra, rec, eps2);
+        } else {
+            finish<COUNT>(A, idx, n, ax_out, ay_out, az_out, err);
+            const int p = q.take();
+            if (p >= 0)
+                A.start(rec, p, lane, n);
+            else
+                run_a = false;
         }
         if ((rb.link.z & 0xFF) == KIND_INTERNAL) {
             if (COUNT)
                 ++B.visits;
             advance_internal(B, rb, act_b, acc_b);
-        } else if (B.i >= wcount) {
-            run_b = false;
-        } else {
+        } else if (B.i < wcount) {
             if (COUNT)
                 ++B.visits;
             visit_rare<SOFT, COUNT>(B, rb, rec, eps2);
+        } else {
+            finish<COUNT>(B, idx, n, ax_out, ay_out, az_out, err);
+            const int p = q.take();
+            if (p >= 0)
+                B.start(rec, p, lane, n);
+            else
+                run_b = false;
         }
-        (void)rare;
     }
-    if (run_a)
+    int fold = 0;
+    if (run_a) {
         walk_rest<CONTAINS, SOFT, COUNT>(A, wrec, wcenter, rec, wcount, eps2, fold);
-    if (run_b)
+        finish<COUNT>(A, idx, n, ax_out, ay_out, az_out, err);
+    }
+    if (run_b) {
         walk_rest<CONTAINS, SOFT, COUNT>(B, wrec, wcenter, rec, wcount, eps2, fold);
-    finish<COUNT>(A, idx, n, ax_out, ay_out, az_out, err);
-    if (has_b)
         finish<COUNT>(B, idx, n, ax_out, ay_out, az_out, err);
+    }
     if (COUNT) {
         unsigned long long npp = A.npp + B.npp, npn = A.npn + B.npn;
         for (int o = 32; o > 0; o >>= 1) {
@@ -438,29 +517,44 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     }
 }
 
+__global__ void queue_clear_kernel(int* __restrict__ queue)
+{
+    if (threadIdx.x < 8)
+        queue[threadIdx.x * kQueueStride] = 0;
+}
+
 } // namespace
 
-// Blocks of four wavefronts, two packets each, padded to a multiple of the
-// eight XCDs.
-static int walk_pair_blocks_padded(int n)
+// One wavefront per resident slot: 32 per CU (8 per SIMD), fewer when
+// there are fewer packet pairs.
+static int walk_persistent_blocks(int n)
 {
-    const int packets = (n + 63) / 64;
-    const int pairs = (packets + 1) / 2;
-    const int blocks = (pairs + 3) / 4;
-    return (blocks + 7) / 8 * 8;
+    static int cus[64] = {};
+    int dev = 0;
+    NBH_HIP(hipGetDevice(&dev));
+    int& c = cus[dev & 63];
+    if (c == 0)
+        NBH_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
+    const int pairs = ((n + 63) / 64 + 1) / 2;
+    const int waves = std::min(pairs, 32 * c);
+    return (waves + 3) / 4;
 }
 
 void launch_walk_mixed(hipStream_t s, const int4* wpairs,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx, int n,
                        float eps2, bool contains, bool soft, double* ax,
-                       double* ay, double* az, int* err,
+                       double* ay, double* az, int* queue, int* err,
                        unsigned long long* counters)
 {
-    const int nb = walk_pair_blocks_padded(n);
+    queue_clear_kernel<<<1, 64, 0, s>>>(queue);
+    NBH_HIP(hipGetLastError());
+    const int nb = walk_persistent_blocks(n);
+    if (nb == 0)
+        return;
     const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
 #define NBH_ARGS                                                                 \
-    wrec, wcenter, d_wcount, rec, idx, n, nb, eps2, ax, ay, az, err,       \
+    wrec, wcenter, d_wcount, rec, idx, n, queue, eps2, ax, ay, az, err,    \
         counters
 #define NBH_LAUNCH(C, S, K)                                                      \
     walk_mixed_kernel<C, S, K><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS)
