@@ -253,7 +253,8 @@ void launch_walk_fp64(hipStream_t s, const double4* wnode, const double* wcrit,
 // wpairs: interleaved {node, link} int4 pairs, 32 bytes per position.
 // queue: kWalkQueueInts ints of work-queue counters (cleared by the launch).
 constexpr int kWalkQueueInts = 256;
-void launch_walk_mixed(hipStream_t s, const int4* wpairs,
+// wcap: records the wpairs buffer holds.
+void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx, int n,
                        float eps2, bool contains, bool soft, double* ax,

@@ -228,7 +228,7 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters)
     if (mixed(s)) {
         const double scale = 2147483648.0 / s->rb->box[3];
         const double eq = eps * scale;
-        launch_walk_mixed(st, s->w32node.p, s->w32center.p,
+        launch_walk_mixed(st, s->w32node.p, s->mcap, s->w32center.p,
                           s->scalars.p + 3, s->rec32.p, s->idx.p, ni,
                           static_cast<float>(eq * eq), contains, eps > 0.0,
                           a[0].p, a[1].p, a[2].p, s->walkq.p,

@@ -258,6 +258,38 @@ __device__ __forceinline__ void load_pair(const WalkRec* pa, const WalkRec* pb,
     rb = as_record(b);
 }
 
+// The same with 32-bit byte offsets from the record array's base, used
+// while the records span less than 2 GiB: each address is then one shift.
+__device__ __forceinline__ void load_pair_off(const WalkRec* base, unsigned oa,
+                                              unsigned ob, WalkRec& ra, WalkRec& rb)
+{
+    int8v a, b;
+    asm volatile("s_load_dwordx8 %0, %2, %3\n\t"
+                 "s_load_dwordx8 %1, %2, %4\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&s"(a), "=&s"(b)
+                 : "s"(base), "s"(oa), "s"(ob)
+                 : "memory");
+    ra = as_record(a);
+    rb = as_record(b);
+}
+
+// Buffer resource over the record array (raw, byte-addressed; word 3 is
+// the gfx9 default format).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t records_rsrc(const WalkRec* base,
+                                                              unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<WalkRec*>(base), 0,
+                                             static_cast<int>(bytes), 0x00020000);
+}
+
+// The skip link of the record at byte offset `o`, through the vector memory
+// path into a vector register (see skip_vgpr).
+__device__ __forceinline__ int skip_vgpr_off(__amdgpu_buffer_rsrc_t rsrc, unsigned o)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(rsrc, 20, static_cast<int>(o), 0);
+}
+
 // Next position after an internal-node visit: the active lanes that opened
 // the node are still at or before i (skip > i, so an accepting lane has
 // left).
@@ -411,9 +443,9 @@ __device__ __forceinline__ int xcd_id()
 // takes the next packet from the queue, so the slots stay paired until the
 // queue is empty; the remaining slot then finishes alone.  Every wavefront
 // leaves once the queue is empty and its slots are done.
-template <bool CONTAINS, bool SOFT, bool COUNT>
+template <bool CONTAINS, bool SOFT, bool COUNT, bool WIDE>
 __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
-    const WalkRec* __restrict__ wrec,
+    const WalkRec* __restrict__ wrec, unsigned wbytes,
     const int4* __restrict__ wcenter, const int* __restrict__ d_wcount,
     const int4* __restrict__ rec, const uint32_t* __restrict__ idx, int n,
     int* __restrict__ queue, float eps2, double* __restrict__ ax_out,
@@ -427,6 +459,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     if (pa < 0)
         return;
     const int wcount = *d_wcount;
+    const __amdgpu_buffer_rsrc_t rsrc = records_rsrc(wrec, WIDE ? 0u : wbytes);
     Walker A, B;
     A.npp = A.npn = A.visits = 0;
     B.npp = B.npn = B.visits = 0;
@@ -440,15 +473,27 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
         // branch besides its exit, so both record loads issue together.
         WalkRec ra, rb;
         unsigned long long act_a, acc_a, act_b, acc_b;
-        int left = kFold;
+        static_assert((kFold & (kFold - 1)) == 0, "kFold must be a power of two");
+        unsigned done = 0; // iterations of this fold period
         for (;;) {
-            const int sa = skip_vgpr(wrec + A.i);
-            const int sb = skip_vgpr(wrec + B.i);
-            load_pair(wrec + A.i, wrec + B.i, ra, rb);
+            int sa, sb;
+            if (WIDE) {
+                sa = skip_vgpr(wrec + A.i);
+                sb = skip_vgpr(wrec + B.i);
+                load_pair(wrec + A.i, wrec + B.i, ra, rb);
+            } else {
+                const unsigned oa = static_cast<unsigned>(A.i) << 5;
+                const unsigned ob = static_cast<unsigned>(B.i) << 5;
+                sa = skip_vgpr_off(rsrc, oa);
+                sb = skip_vgpr_off(rsrc, ob);
+                load_pair_off(wrec, oa, ob, ra, rb);
+            }
             visit_mac<CONTAINS, COUNT, false>(A, ra, sa, wcenter, eps2, act_a, acc_a);
             visit_mac<CONTAINS, COUNT, false>(B, rb, sb, wcenter, eps2, act_b, acc_b);
-            const bool rare = ((ra.link.z | rb.link.z) & 0xFF) != KIND_INTERNAL;
-            if (rare || --left == 0)
+            // One exit test in integer arithmetic: a rare record kind in
+            // either slot, or the end of the fold period.
+            ++done;
+            if ((((ra.link.z | rb.link.z) & 0xFF) | (done & kFold)) != 0)
                 break;
             if (COUNT) {
                 ++A.visits;
@@ -540,13 +585,16 @@ static int walk_persistent_blocks(int n)
     return (waves + 3) / 4;
 }
 
-void launch_walk_mixed(hipStream_t s, const int4* wpairs,
+void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx, int n,
                        float eps2, bool contains, bool soft, double* ax,
                        double* ay, double* az, int* queue, int* err,
                        unsigned long long* counters)
 {
+    // 32-bit record offsets while the record array spans less than 2 GiB
+    const size_t wbytes = wcap * sizeof(WalkRec);
+    const bool wide = wbytes > 0x7FFFFFFFull;
     queue_clear_kernel<<<1, 64, 0, s>>>(queue);
     NBH_HIP(hipGetLastError());
     const int nb = walk_persistent_blocks(n);
@@ -554,10 +602,15 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs,
         return;
     const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
 #define NBH_ARGS                                                                 \
-    wrec, wcenter, d_wcount, rec, idx, n, queue, eps2, ax, ay, az, err,    \
-        counters
+    wrec, static_cast<unsigned>(wide ? 0 : wbytes), wcenter, d_wcount, rec, idx, \
+        n, queue, eps2, ax, ay, az, err, counters
 #define NBH_LAUNCH(C, S, K)                                                      \
-    walk_mixed_kernel<C, S, K><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS)
+    do {                                                                         \
+        if (wide)                                                                \
+            walk_mixed_kernel<C, S, K, true><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS); \
+        else                                                                     \
+            walk_mixed_kernel<C, S, K, false><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS); \
+    } while (0)
     const bool cnt = counters != nullptr;
     if (contains) {
         if (soft) {
