@@ -18,8 +18,10 @@
 // without changing any lane's interaction list.
 //
 // Accumulation.  Each lane adds its interactions into FP32 partial sums and
-// folds them into a Kahan-compensated sum every kFold visits, so the
-// compensated total carries the rounding of at most kFold terms per fold.
+// folds them into a Kahan-compensated sum after every kFold visits of its
+// packet's walk, so the compensated total carries the rounding of at most
+// kFold terms per fold, and a target's result does not depend on the
+// schedule.
 #include "walk_common.hpp"
 
 #include <algorithm>
@@ -28,7 +30,7 @@ namespace nbh {
 
 namespace {
 
-constexpr int kFold = 32;
+constexpr int kFold = 64;
 // ints between the work-queue counters of two ranges (one per 128 bytes)
 constexpr int kQueueStride = kWalkQueueInts / 8;
 
@@ -121,11 +123,15 @@ struct Walker {
     int next;
     int code;      // traversal error of this lane, 0 if none
     int i;         // current position (wave-uniform)
+    unsigned cnt;  // visits since the packet's last fold (wave-uniform)
     Acc a;
     unsigned long long npp, npn, visits;
 
     // Starts the walk of `packet`; the interaction and visit counts go on
-    // accumulating over the packets a slot walks.
+    // accumulating over the packets a slot walks.  The partial sums are
+    // folded after every kFold-th visit of the packet's own walk, so the
+    // rounding of a target's sum does not depend on which wavefront or slot
+    // walked it, or with which other packet.
     __device__ __forceinline__ void start(const int4* __restrict__ rec, int packet,
                                           int lane, int n)
     {
@@ -135,6 +141,7 @@ struct Walker {
         next = valid ? 0 : kIdle;
         code = 0;
         i = 0;
+        cnt = 0;
         a = Acc();
     }
 };
@@ -170,9 +177,43 @@ __device__ __forceinline__ int select_i(unsigned long long m, int a, int b)
     return r;
 }
 
+// Adds the monopole f = gm * inv3 (per lane) times (dx, dy, dz) to
+// (x, y, z) in the lanes of m only: the statement narrows exec to m for the
+// multiply and the three fused adds and restores it, so the other lanes
+// need no zeroing select.
+__device__ __forceinline__ void add_in_lanes(unsigned long long m, float gm,
+                                             float inv3, float dx, float dy,
+                                             float dz, float& x, float& y,
+                                             float& z)
+{
+    unsigned long long saved;
+    float f;
+    asm volatile("s_and_saveexec_b64 %[sv], %[m]\n\t"
+                 "v_mul_f32_e32 %[f], %[gm], %[i3]\n\t"
+                 "v_fmac_f32_e32 %[x], %[f], %[dx]\n\t"
+                 "v_fmac_f32_e32 %[y], %[f], %[dy]\n\t"
+                 "v_fmac_f32_e32 %[z], %[f], %[dz]\n\t"
+                 "s_mov_b64 exec, %[sv]"
+                 : [x] "+v"(x), [y] "+v"(y), [z] "+v"(z), [sv] "=&s"(saved),
+                   [f] "=&v"(f)
+                 : [m] "s"(m), [gm] "s"(gm), [i3] "v"(inv3), [dx] "v"(dx),
+                   [dy] "v"(dy), [dz] "v"(dz)
+                 : "scc"); // s_and_saveexec sets SCC
+}
+
 __device__ __forceinline__ bool lane_bit(unsigned long long m)
 {
     return ((m >> (threadIdx.x & 63)) & 1ull) != 0ull;
+}
+
+// Ends a visit: folds the partial sums after the packet's kFold-th visit
+// since the last fold (`cnt` was incremented for this visit).
+__device__ __forceinline__ void end_visit(Walker& w)
+{
+    if (w.cnt == kFold) {
+        w.a.fold();
+        w.cnt = 0;
+    }
 }
 
 // The straight-line part of a visit, for records of every kind: the MAC
@@ -215,10 +256,8 @@ __device__ __forceinline__ void visit_mac(Walker& w, const WalkRec& r, int skipv
     }
     if (!BRANCH || accm != 0ull) {
         const float inv = __builtin_amdgcn_rsqf(r2);
-        const float f = select_f(accm, 0.0f, __int_as_float(nd.w) * (inv * inv * inv));
-        w.a.px = fmaf(f, dx, w.a.px);
-        w.a.py = fmaf(f, dy, w.a.py);
-        w.a.pz = fmaf(f, dz, w.a.pz);
+        add_in_lanes(accm, __int_as_float(nd.w), inv * inv * inv, dx, dy, dz,
+                     w.a.px, w.a.py, w.a.pz);
         if (COUNT && lane_bit(accm) && __int_as_float(nd.w) != 0.0f) {
             if ((lk.z >> 8) == 1) {
                 if (lk.w != w.t)
@@ -346,7 +385,7 @@ template <bool CONTAINS, bool SOFT, bool COUNT>
 __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__ wrec,
                                           const int4* __restrict__ wcenter,
                                           const int4* __restrict__ rec,
-                                          int wcount, float eps2, int& fold)
+                                          int wcount, float eps2)
 {
     for (;;) {
         WalkRec r;
@@ -358,10 +397,8 @@ __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__
                 break;
             if (COUNT)
                 ++w.visits;
-            if (++fold == kFold) {
-                w.a.fold();
-                fold = 0;
-            }
+            ++w.cnt;
+            end_visit(w);
             advance_internal(w, r, actm, accm);
         }
         if (w.i >= wcount)
@@ -369,10 +406,8 @@ __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__
         if (COUNT)
             ++w.visits;
         visit_rare<SOFT, COUNT>(w, r, rec, eps2);
-        if (++fold == kFold) {
-            w.a.fold();
-            fold = 0;
-        }
+        ++w.cnt;
+        end_visit(w);
     }
 }
 
@@ -469,12 +504,12 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     if (run_b)
         B.start(rec, pb, lane, n);
     while (run_a && run_b) {
-        // kFold lock-step iterations between folds; the loop body has no
-        // branch besides its exit, so both record loads issue together.
+        // Lock-step iterations until a slot reaches a rare record kind, its
+        // end, or its fold point; the loop body has no branch besides its
+        // exit, so both record loads issue together.
         WalkRec ra, rb;
         unsigned long long act_a, acc_a, act_b, acc_b;
         static_assert((kFold & (kFold - 1)) == 0, "kFold must be a power of two");
-        unsigned done = 0; // iterations of this fold period
         for (;;) {
             int sa, sb;
             if (WIDE) {
@@ -490,10 +525,11 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             }
             visit_mac<CONTAINS, COUNT, false>(A, ra, sa, wcenter, eps2, act_a, acc_a);
             visit_mac<CONTAINS, COUNT, false>(B, rb, sb, wcenter, eps2, act_b, acc_b);
-            // One exit test in integer arithmetic: a rare record kind in
-            // either slot, or the end of the fold period.
-            ++done;
-            if ((((ra.link.z | rb.link.z) & 0xFF) | (done & kFold)) != 0)
+            ++A.cnt;
+            ++B.cnt;
+            // One exit test in integer arithmetic (cnt < kFold, a power of
+            // two, until a fold point).
+            if ((((ra.link.z | rb.link.z) & 0xFF) | ((A.cnt | B.cnt) & kFold)) != 0)
                 break;
             if (COUNT) {
                 ++A.visits;
@@ -502,18 +538,18 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             advance_internal(A, ra, act_a, acc_a);
             advance_internal(B, rb, act_b, acc_b);
         }
-        A.a.fold();
-        B.a.fold();
         // Finish both visits of the last iteration; a slot at the end of
         // its walk moves on to the next packet.
         if ((ra.link.z & 0xFF) == KIND_INTERNAL) {
             if (COUNT)
                 ++A.visits;
+            end_visit(A);
             advance_internal(A, ra, act_a, acc_a);
         } else if (A.i < wcount) {
             if (COUNT)
                 ++A.visits;
             visit_rare<SOFT, COUNT>(A, ra, rec, eps2);
+            end_visit(A);
         } else {
             finish<COUNT>(A, idx, n, ax_out, ay_out, az_out, err);
             const int p = q.take();
@@ -525,11 +561,13 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
         if ((rb.link.z & 0xFF) == KIND_INTERNAL) {
             if (COUNT)
                 ++B.visits;
+            end_visit(B);
             advance_internal(B, rb, act_b, acc_b);
         } else if (B.i < wcount) {
             if (COUNT)
                 ++B.visits;
             visit_rare<SOFT, COUNT>(B, rb, rec, eps2);
+            end_visit(B);
         } else {
             finish<COUNT>(B, idx, n, ax_out, ay_out, az_out, err);
             const int p = q.take();
@@ -539,13 +577,12 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
                 run_b = false;
         }
     }
-    int fold = 0;
     if (run_a) {
-        walk_rest<CONTAINS, SOFT, COUNT>(A, wrec, wcenter, rec, wcount, eps2, fold);
+        walk_rest<CONTAINS, SOFT, COUNT>(A, wrec, wcenter, rec, wcount, eps2);
         finish<COUNT>(A, idx, n, ax_out, ay_out, az_out, err);
     }
     if (run_b) {
-        walk_rest<CONTAINS, SOFT, COUNT>(B, wrec, wcenter, rec, wcount, eps2, fold);
+        walk_rest<CONTAINS, SOFT, COUNT>(B, wrec, wcenter, rec, wcount, eps2);
         finish<COUNT>(B, idx, n, ax_out, ay_out, az_out, err);
     }
     if (COUNT) {

@@ -91,6 +91,55 @@ def test_mixed_pipeline_accuracy(nb, orc, theta, kind):
     assert rms < MIXED_RMS and mx < MIXED_MAX
 
 
+@pytest.mark.parametrize("n", [2, 63, 64, 65, 129, 1091])
+def test_mixed_walk_ragged_sizes(nb, orc, n):
+    """Packet counts that leave a wavefront's second slot empty, fewer packets
+    than wavefronts, and a last packet with idle lanes."""
+    p = make(nb, "plummer", n, seed=7)
+    sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
+                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    ref = orc.Sim(p, 1e-4, 0.5, 1e-3, 1.0, 21)
+    sim.initialize()
+    ref.initialize()
+    rs = ref.state()
+    rms, mx = rel_err(sim.accelerations(), [rs["ax"], rs["ay"], rs["az"]])
+    assert rms < MIXED_RMS and mx < MIXED_MAX
+
+
+def test_mixed_walk_single_particle_is_zero(nb):
+    p = make(nb, "plummer", 1)
+    sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
+                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    sim.initialize()
+    for a in sim.accelerations():
+        assert np.array_equal(a, np.zeros(1))
+
+
+def test_mixed_walk_independent_of_schedule(nb):
+    """Wavefronts take packets from a queue in whatever order they run; each
+    packet's sums depend only on its targets, so two simulations of the same
+    input, and the same state walked again, agree bit for bit."""
+    p = make(nb, "clumps", 50_000, seed=3)
+    sims = [nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
+                          precision=nb.PRECISION_MIXED, bits_per_axis=21)
+            for _ in range(2)]
+    for s in sims:
+        s.initialize()
+    a0, a1 = sims[0].accelerations(), sims[1].accelerations()
+    for x, y in zip(a0, a1):
+        assert np.array_equal(x, y)
+    for s in sims:
+        s.run(2)
+    a0, a1 = sims[0].accelerations(), sims[1].accelerations()
+    for x, y in zip(a0, a1):
+        assert np.array_equal(x, y)
+    st0, st1 = sims[0].state(), sims[1].state()
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(st0[f], st1[f]), f
+    for s in sims:
+        s.close()
+
+
 def test_mixed_pipeline_without_softening(nb, orc):
     p = make(nb, "uniform", 3000)
     sim = nb.Simulation(p, 1e-4, 0.5, 0.0, gravitational_constant=1.0,
