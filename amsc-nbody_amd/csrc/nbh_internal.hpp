@@ -254,9 +254,11 @@ void launch_walk_fp64(hipStream_t s, const double4* wnode, const double* wcrit,
 // queue: kWalkQueueInts ints of work-queue counters (cleared by the launch).
 constexpr int kWalkQueueInts = 256;
 // wcap: records the wpairs buffer holds.
+// order: sorted target indices in the packets' order (null: Morton order).
 void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
-                       const int4* rec, const uint32_t* idx, int n,
+                       const int4* rec, const uint32_t* idx,
+                       const uint32_t* order, int n,
                        float eps2, bool contains, bool soft, double* ax,
                        double* ay, double* az, int* queue, int* err,
                        unsigned long long* counters);

@@ -14,6 +14,7 @@
 // groups, Karras, plan, octree, mass/COM, walk layout), forces into the next
 // buffer, closing kick, swap.
 #include "sim_internal.hpp"
+#include "hilbert.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -46,8 +47,14 @@ void alloc_state(nbh_sim* s)
     s->idx_alt.alloc(n);
     s->rec64.alloc(n);
     s->pos4.alloc(n);
-    if (mixed(s))
+    if (mixed(s)) {
         s->rec32.alloc(n);
+        s->horder.alloc(n);
+        const HilbertTables& tab = hilbert_tables();
+        s->htab.alloc(sizeof(HilbertTables));
+        NBH_HIP(hipMemcpyAsync(s->htab.p, &tab, sizeof(HilbertTables),
+                               hipMemcpyHostToDevice, s->stream));
+    }
     s->gfirst.alloc(n);
     s->gcount.alloc(n);
     s->left.alloc(n);
@@ -107,6 +114,9 @@ void ensure_octree(nbh_sim* s, size_t m)
                     &s->cenz, &s->half, &s->crit})
         b->alloc(cap);
     if (mixed(s)) {
+        s->hcount.alloc(cap);
+        s->hoffset.alloc(cap);
+        s->hstate.alloc(cap);
         s->w32node.alloc(2 * cap); // interleaved {node, link} pairs
         if (s->cfg.theta >= kContainsTheta)
             s->w32center.alloc(cap);
@@ -213,7 +223,8 @@ void rebuild(nbh_sim* s, bool kick_drift)
         s->cfg.softening, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->sum4.p, s->poffset.p, s->pcount.p, s->ngroups,
         s->w64node.p, s->w64crit.p,
         s->w64link.p, s->w64center.p, s->w32node.p, nullptr,
-        s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr);
+        s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr,
+        s->hcount.p, s->hoffset.p, s->hstate.p, s->horder.p, s->htab.p);
 }
 
 void forces(nbh_sim* s, int buffer, unsigned long long* counters)
@@ -229,7 +240,7 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters)
         const double scale = 2147483648.0 / s->rb->box[3];
         const double eq = eps * scale;
         launch_walk_mixed(st, s->w32node.p, s->mcap, s->w32center.p,
-                          s->scalars.p + 3, s->rec32.p, s->idx.p, ni,
+                          s->scalars.p + 3, s->rec32.p, s->idx.p, s->horder.p, ni,
                           static_cast<float>(eq * eq), contains, eps > 0.0,
                           a[0].p, a[1].p, a[2].p, s->walkq.p,
                           s->errors + ERR_TRAVERSAL, counters);
@@ -553,6 +564,7 @@ int nbh_sim_structure_view(nbh_sim* s, nbh_sim_structure* out)
         out->radix_level = s->plevel.p;
         out->edge_node_count = s->pcount.p;
         out->edge_node_offset = s->poffset.p;
+        out->target_order = mixed(s) ? s->horder.p : nullptr;
         out->box = s->box.p;
     });
 }

@@ -31,6 +31,7 @@
 // target's acceleration with Kahan compensation.
 #include "octree_device.hpp"
 #include "walk_common.hpp"
+#include "hilbert.hpp"
 
 namespace nbh {
 
@@ -122,6 +123,17 @@ __device__ __forceinline__ int preorder(const nbh_octree_view& o,
 
 // --------------------------------------------------------- walk records
 // Outputs of the walk build (records indexed by pre-order position).
+// The mixed traversal's target order along the Hilbert curve (see
+// hilbert.cpp), built alongside the level passes; all pointers are null
+// when the order is not wanted.
+struct HilbertOut {
+    int* count;          // subtree particle count per node (bottom-up)
+    int* offset;         // curve rank of the subtree's first particle
+    uint8_t* state;      // curve orientation state per node (top-down)
+    uint32_t* order;     // curve rank -> sorted particle index
+    const uint8_t* tab;  // HilbertTables: rank[24][8] then next[24][8]
+};
+
 struct WalkOut {
     double4* w64node;
     double* w64crit;
@@ -305,7 +317,7 @@ __global__ __launch_bounds__(256) void level_up_kernel(
     const double* __restrict__ box, int* __restrict__ size,
     int* __restrict__ local, double* __restrict__ crit, bool collapse,
     const int* __restrict__ order, const int* __restrict__ start,
-    double4* __restrict__ sum4)
+    double4* __restrict__ sum4, HilbertOut h)
 {
     using Key = typename Layout<BITS>::Key;
     const double rcx = box[0], rcy = box[1], rcz = box[2], side = box[3];
@@ -332,6 +344,8 @@ __global__ __launch_bounds__(256) void level_up_kernel(
                 wz = fma(p.w, p.z, wz);
             }
             size[v] = 1;
+            if (h.count)
+                h.count[v] = pc;
         } else {
             const int4 c0 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v)];
             const int4 c1 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v) + 1];
@@ -353,6 +367,14 @@ __global__ __launch_bounds__(256) void level_up_kernel(
             // the node's record on the next top-down level.  Only internal
             // children and single-particle leaves qualify (a leaf holding
             // several particles is summed exactly, never as a monopole).
+            if (h.count) {
+                int hc = 0;
+#pragma unroll
+                for (int oc = 0; oc < 8; ++oc)
+                    if (ch[oc] != -1)
+                        hc += h.count[ch[oc]];
+                h.count[v] = hc;
+            }
             int off = (collapse && nch == 1 && o.particle_count[only] <= 1) ? 0 : 1;
 #pragma unroll
             for (int oc = 7; oc >= 0; --oc) {
@@ -385,6 +407,40 @@ __global__ __launch_bounds__(256) void level_up_kernel(
 
 // Top-down over one level: pre-order position = parent's + own offset, then
 // the node's walk record.
+// Curve offsets and states of node v's children: the children in curve
+// order (their ranks in v's orientation state) take consecutive particle
+// ranges from v's offset.  The root is in state 0 at offset 0.
+__device__ __forceinline__ void curve_children(const nbh_octree_view& o, int v, int L,
+                                               const HilbertOut& h)
+{
+    const int st = (L == 0) ? 0 : h.state[v];
+    const int off = (L == 0) ? 0 : h.offset[v];
+    if (o.particle_count[v] > 0)
+        return;
+    const int4 c0 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v)];
+    const int4 c1 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v) + 1];
+    const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    const uint8_t* rank = h.tab + 8 * st;
+    const uint8_t* next = h.tab + 8 * kHilbertStates + 8 * st;
+    int cnt[8], rk[8];
+#pragma unroll
+    for (int oc = 0; oc < 8; ++oc) {
+        cnt[oc] = ch[oc] >= 0 ? h.count[ch[oc]] : 0;
+        rk[oc] = rank[oc];
+    }
+#pragma unroll
+    for (int oc = 0; oc < 8; ++oc) {
+        if (ch[oc] < 0)
+            continue;
+        int before = off;
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            before += (rk[b] < rk[oc]) ? cnt[b] : 0;
+        h.offset[ch[oc]] = before;
+        h.state[ch[oc]] = next[oc];
+    }
+}
+
 template <int BITS, bool MIXED>
 __global__ __launch_bounds__(256) void level_down_kernel(
     nbh_octree_view o, int L, const int* __restrict__ size,
@@ -393,7 +449,8 @@ __global__ __launch_bounds__(256) void level_down_kernel(
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
     const double* __restrict__ box, double G, bool need_center, double eps,
     WalkOut w, int* __restrict__ wcount, const int* __restrict__ order,
-    const int* __restrict__ start, const double4* __restrict__ sum4)
+    const int* __restrict__ start, const double4* __restrict__ sum4,
+    HilbertOut h)
 {
     using Key = typename Layout<BITS>::Key;
     const Key* prefix = static_cast<const Key*>(o.prefix);
@@ -402,6 +459,8 @@ __global__ __launch_bounds__(256) void level_down_kernel(
     for (int k = start[L] + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
          k < end; k += stride) {
         const int v = order[k];
+        if (h.count)
+            curve_children(o, v, L, h);
         int pre = 0;
         if (L == 0) {
             const int wc = size[0] < o.n_nodes ? size[0] : o.n_nodes;
@@ -586,7 +645,7 @@ __global__ __launch_bounds__(256) void leaf_up_kernel(
     nbh_octree_view o, const int* __restrict__ poffset,
     const int* __restrict__ pcount, int n_groups,
     const double4* __restrict__ rec, double4* __restrict__ sum4,
-    int* __restrict__ size)
+    int* __restrict__ size, HilbertOut h)
 {
     const int g = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (g >= n_groups)
@@ -613,6 +672,8 @@ __global__ __launch_bounds__(256) void leaf_up_kernel(
     }
     sum4[v] = make_double4(cx, cy, cz, tm);
     size[v] = 1;
+    if (h.count)
+        h.count[v] = pc;
 }
 
 // Leaf positions and records, after every internal level has its position.
@@ -624,7 +685,7 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
     const double* __restrict__ crit, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
     const double* __restrict__ box, double G, bool need_center, double eps,
-    WalkOut w, const double4* __restrict__ sum4)
+    WalkOut w, const double4* __restrict__ sum4, HilbertOut h)
 {
     using Key = typename Layout<BITS>::Key;
     const int g = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -633,6 +694,15 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
     const int v = leaf_of_group(poffset, pcount, o, g);
     if (v < 0)
         return;
+    if (h.order) {
+        // the leaf's particles, in sorted order, at its curve offset
+        const int off = (v == 0) ? 0 : h.offset[v];
+        const int f = o.first_particle[v];
+        const int pc = o.particle_count[v];
+        if (off >= 0 && pc > 0 && off <= o.n_particles - pc)
+            for (int k = 0; k < pc; ++k)
+                h.order[off + k] = static_cast<uint32_t>(f + k);
+    }
     const int parent = o.parent[v];
     if (parent < 0 || parent >= o.n_nodes)
         return;
@@ -721,11 +791,13 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          double4* w64node, double* w64crit,
                          int4* w64link, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
-                         hipEvent_t between)
+                         hipEvent_t between, int* hcount, int* hoffset,
+                         uint8_t* hstate, uint32_t* horder, const uint8_t* htab)
 {
     // Mixed records are interleaved {node, link} pairs: w32node is the array
     // base and w32link = w32node + 1, both indexed at 2 * position.
     (void)w32link;
+    const HilbertOut h{hcount, hoffset, hstate, horder, htab};
     int4* w32pair_link = w32node ? w32node + 1 : nullptr;
     // Single-child nodes are collapsed into their child in the mixed walk:
     // they carry the child's mass and centre of mass, and for theta below
@@ -749,12 +821,12 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     const int* start = level_info;
     const unsigned gblocks = blocks_for(n_groups > 0 ? n_groups : 1, 256);
     leaf_up_kernel<<<gblocks, 256, 0, s>>>(o, poffset, pcount, n_groups, rec64, sum4,
-                                           size);
+                                           size, h);
     NBH_HIP(hipGetLastError());
     for (int L = max_level; L >= 0; --L) {
         level_up_kernel<BITS><<<blocks, 256, 0, s>>>(o, L, rec64, theta, box, size,
                                                      local, crit, collapse, order,
-                                                     start, sum4);
+                                                     start, sum4, h);
         NBH_HIP(hipGetLastError());
     }
     if (between)
@@ -771,11 +843,11 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
         if (mixed)
             level_down_kernel<BITS, true><<<blocks, 256, 0, s>>>(
                 o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
-                need_center, eps, w, wcount, order, start, sum4);
+                need_center, eps, w, wcount, order, start, sum4, h);
         else
             level_down_kernel<BITS, false><<<blocks, 256, 0, s>>>(
                 o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
-                need_center, eps, w, wcount, order, start, sum4);
+                need_center, eps, w, wcount, order, start, sum4, h);
         NBH_HIP(hipGetLastError());
     }
     // Leaves last: a leaf that collapsed its single-child parent overwrites
@@ -783,11 +855,11 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     if (mixed)
         leaf_down_kernel<BITS, true><<<gblocks, 256, 0, s>>>(
             o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
-            box, G, need_center, eps, w, sum4);
+            box, G, need_center, eps, w, sum4, h);
     else
         leaf_down_kernel<BITS, false><<<gblocks, 256, 0, s>>>(
             o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
-            box, G, need_center, eps, w, sum4);
+            box, G, need_center, eps, w, sum4, h);
     NBH_HIP(hipGetLastError());
 }
 
@@ -830,7 +902,8 @@ template void launch_octree_views<21>(hipStream_t, const nbh_octree_view&,
                                          double*, int*, int*, int*, double4*,           \
                                          const int*, const int*, int, double4*,          \
                                          double*, int4*,                                 \
-                                         double4*, int4*, int4*, int4*, int*, hipEvent_t);
+                                         double4*, int4*, int4*, int4*, int*, hipEvent_t, \
+                                         int*, int*, uint8_t*, uint32_t*, const uint8_t*);
 NBH_LEVEL_INST(10)
 NBH_LEVEL_INST(21)
 #undef NBH_LEVEL_INST

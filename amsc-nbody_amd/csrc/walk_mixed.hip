@@ -127,19 +127,32 @@ struct Walker {
     Acc a;
     unsigned long long npp, npn, visits;
 
-    // Starts the walk of `packet`; the interaction and visit counts go on
+    // Starts the walk of `packet`: targets order[packet * 64 + lane] (sorted
+    // indices along the Hilbert curve), or the Morton-consecutive ones when
+    // there is no order.  The interaction and visit counts go on
     // accumulating over the packets a slot walks.  The partial sums are
     // folded after every kFold-th visit of the packet's own walk, so the
     // rounding of a target's sum does not depend on which wavefront or slot
     // walked it, or with which other packet.
-    __device__ __forceinline__ void start(const int4* __restrict__ rec, int packet,
-                                          int lane, int n)
+    __device__ __forceinline__ void start(const int4* __restrict__ rec,
+                                          const uint32_t* __restrict__ order,
+                                          int packet, int lane, int n)
     {
-        t = packet * 64 + lane;
-        valid = t < n;
+        const int k = packet * 64 + lane;
+        valid = k < n;
+        t = k;
+        code = 0;
+        if (valid && order) {
+            const uint32_t o = order[k];
+            if (o < static_cast<uint32_t>(n)) {
+                t = static_cast<int>(o);
+            } else {
+                valid = false;
+                code = 3;
+            }
+        }
         tq = valid ? rec[t] : make_int4(0, 0, 0, 0);
         next = valid ? 0 : kIdle;
-        code = 0;
         i = 0;
         cnt = 0;
         a = Acc();
@@ -482,7 +495,8 @@ template <bool CONTAINS, bool SOFT, bool COUNT, bool WIDE>
 __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const WalkRec* __restrict__ wrec, unsigned wbytes,
     const int4* __restrict__ wcenter, const int* __restrict__ d_wcount,
-    const int4* __restrict__ rec, const uint32_t* __restrict__ idx, int n,
+    const int4* __restrict__ rec, const uint32_t* __restrict__ idx,
+    const uint32_t* __restrict__ order, int n,
     int* __restrict__ queue, float eps2, double* __restrict__ ax_out,
     double* __restrict__ ay_out, double* __restrict__ az_out, int* err,
     unsigned long long* __restrict__ counters)
@@ -498,11 +512,11 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     Walker A, B;
     A.npp = A.npn = A.visits = 0;
     B.npp = B.npn = B.visits = 0;
-    A.start(rec, pa, lane, n);
+    A.start(rec, order, pa, lane, n);
     const int pb = q.take();
     bool run_a = true, run_b = pb >= 0;
     if (run_b)
-        B.start(rec, pb, lane, n);
+        B.start(rec, order, pb, lane, n);
     while (run_a && run_b) {
         // Lock-step iterations until a slot reaches a rare record kind, its
         // end, or its fold point; the loop body has no branch besides its
@@ -554,7 +568,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             finish<COUNT>(A, idx, n, ax_out, ay_out, az_out, err);
             const int p = q.take();
             if (p >= 0)
-                A.start(rec, p, lane, n);
+                A.start(rec, order, p, lane, n);
             else
                 run_a = false;
         }
@@ -572,7 +586,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             finish<COUNT>(B, idx, n, ax_out, ay_out, az_out, err);
             const int p = q.take();
             if (p >= 0)
-                B.start(rec, p, lane, n);
+                B.start(rec, order, p, lane, n);
             else
                 run_b = false;
         }
@@ -624,7 +638,8 @@ static int walk_persistent_blocks(int n)
 
 void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
-                       const int4* rec, const uint32_t* idx, int n,
+                       const int4* rec, const uint32_t* idx,
+                       const uint32_t* order, int n,
                        float eps2, bool contains, bool soft, double* ax,
                        double* ay, double* az, int* queue, int* err,
                        unsigned long long* counters)
@@ -640,7 +655,7 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
     const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
 #define NBH_ARGS                                                                 \
     wrec, static_cast<unsigned>(wide ? 0 : wbytes), wcenter, d_wcount, rec, idx, \
-        n, queue, eps2, ax, ay, az, err, counters
+        order, n, queue, eps2, ax, ay, az, err, counters
 #define NBH_LAUNCH(C, S, K)                                                      \
     do {                                                                         \
         if (wide)                                                                \

@@ -130,7 +130,8 @@ class SimStructure(C.Structure):
                 ("bits_per_axis", _i), ("sorted_keys", _p), ("sorted_indices", _p),
                 ("unique_keys", _p), ("group_first", _p), ("group_count", _p),
                 ("tree", TreeView), ("octree", OctreeView), ("radix_level", _p),
-                ("edge_node_count", _p), ("edge_node_offset", _p), ("box", _p)]
+                ("edge_node_count", _p), ("edge_node_offset", _p), ("box", _p),
+                ("target_order", _p)]
 
 
 def _sig(name, restype, *argtypes):
@@ -994,6 +995,8 @@ class Simulation:
             "n_groups": g,
             "n_nodes": m,
             "box": _copy_d2h(s.box, 4, np.float64),
+            "target_order": (_copy_d2h(s.target_order, n, np.uint32)
+                             if s.target_order else None),
             "children": _copy_d2h(o.children, 8 * m, np.int32),
             "parent": _copy_d2h(o.parent, m, np.int32),
             "level": _copy_d2h(o.level, m, np.int32),

@@ -306,3 +306,56 @@ def truncated_plummer_energy(rmax=10.0, G=1.0):
 
     val, _ = quad(integrand, 0.0, rmax, limit=200)
     return -G * val
+
+
+def hilbert_keys(qx, qy, qz, bits=21):
+    """3-D Hilbert curve indices of integer grid coordinates in [0, 2^bits):
+    J. Skilling's axes-to-transpose transform ("Programming the Hilbert
+    curve", AIP Conf. Proc. 707, 2004), then the transposed bits read from
+    the most significant level down, x, y, z.  Vectorised over arrays."""
+    X = [np.asarray(q, dtype=np.uint64).copy() for q in (qx, qy, qz)]
+    M = np.uint64(1 << (bits - 1))
+    Q = int(M)
+    while Q > 1:
+        q = np.uint64(Q)
+        P = np.uint64(Q - 1)
+        for i in range(3):
+            hi = (X[i] & q) != 0
+            # invert where the bit is set, else exchange the low bits of X0, Xi
+            t = (X[0] ^ X[i]) & P
+            x0_inv = X[0] ^ P
+            x0_ex = X[0] ^ t
+            xi_ex = X[i] ^ t
+            if i == 0:
+                X[0] = np.where(hi, x0_inv, X[0])
+            else:
+                X[0] = np.where(hi, x0_inv, x0_ex)
+                X[i] = np.where(hi, X[i], xi_ex)
+        Q >>= 1
+    X[1] ^= X[0]
+    X[2] ^= X[1]
+    t = np.zeros_like(X[0])
+    Q = int(M)
+    while Q > 1:
+        t ^= np.where((X[2] & np.uint64(Q)) != 0, np.uint64(Q - 1), np.uint64(0))
+        Q >>= 1
+    for i in range(3):
+        X[i] ^= t
+    key = np.zeros_like(X[0])
+    one = np.uint64(1)
+    for b in range(bits - 1, -1, -1):
+        s = np.uint64(b)
+        key = (key << np.uint64(3)) | (((X[0] >> s) & one) << np.uint64(2)) | \
+            (((X[1] >> s) & one) << one) | ((X[2] >> s) & one)
+    return key
+
+
+def morton_decode(keys, bits=21):
+    """Grid coordinates of Morton keys (x in the high bit of each triple)."""
+    k = np.asarray(keys, dtype=np.uint64)
+    q = [np.zeros_like(k) for _ in range(3)]
+    one = np.uint64(1)
+    for b in range(bits):
+        for a in range(3):
+            q[a] |= ((k >> np.uint64(3 * b + 2 - a)) & one) << np.uint64(b)
+    return q

@@ -106,6 +106,35 @@ def test_mixed_walk_ragged_sizes(nb, orc, n):
     assert rms < MIXED_RMS and mx < MIXED_MAX
 
 
+@pytest.mark.parametrize("kind,n", [("plummer", 20_000), ("clumps", 30_000),
+                                    ("uniform", 4096), ("plummer", 1), ("plummer", 65)])
+def test_mixed_target_order_follows_hilbert_curve(nb, kind, n):
+    """The traversal's packets take the sorted particles along the Hilbert
+    curve through their 21-bit grid positions (ties in sorted order): a
+    permutation equal to a stable sort by the curve index."""
+    p = make(nb, kind, n, seed=11)
+    sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
+                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    sim.initialize()
+    s = sim.structure_host()
+    order = s["target_order"]
+    assert order is not None and len(order) == n
+    assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
+    qx, qy, qz = H.morton_decode(s["keys"], 21)
+    hk = H.hilbert_keys(qx, qy, qz, 21)
+    assert np.array_equal(order, np.argsort(hk, kind="stable").astype(np.uint32))
+    sim.close()
+
+
+def test_fp64_has_no_target_order(nb):
+    p = make(nb, "plummer", 1000)
+    sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
+                        precision=nb.PRECISION_FP64, bits_per_axis=21)
+    sim.initialize()
+    assert sim.structure_host()["target_order"] is None
+    sim.close()
+
+
 def test_mixed_walk_single_particle_is_zero(nb):
     p = make(nb, "plummer", 1)
     sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
