@@ -30,30 +30,32 @@ struct Extent {
     double lo[3];
     double hi[3];
     bool bad;
+    double mmax; // largest |m| (0 when masses are not read)
 };
 
 __device__ __forceinline__ void block_reduce_store(Extent e, double* partials)
 {
-    __shared__ double s_red[4][7];
+    __shared__ double s_red[4][8];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    double v[7];
+    double v[8];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         v[a] = wave_min(e.lo[a]);
         v[3 + a] = wave_max(e.hi[a]);
     }
     v[6] = (__ballot(e.bad) != 0ull) ? 1.0 : 0.0;
+    v[7] = wave_max(e.mmax);
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 7; ++k)
+        for (int k = 0; k < 8; ++k)
             s_red[wave][k] = v[k];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double r[7];
+        double r[8];
 #pragma unroll
-        for (int k = 0; k < 7; ++k)
+        for (int k = 0; k < 8; ++k)
             r[k] = s_red[0][k];
         for (int w = 1; w < 4; ++w) {
 #pragma unroll
@@ -62,31 +64,33 @@ __device__ __forceinline__ void block_reduce_store(Extent e, double* partials)
                 r[3 + a] = fmax(r[3 + a], s_red[w][3 + a]);
             }
             r[6] = fmax(r[6], s_red[w][6]);
+            r[7] = fmax(r[7], s_red[w][7]);
         }
 #pragma unroll
-        for (int k = 0; k < 7; ++k)
+        for (int k = 0; k < 8; ++k)
             partials[blockIdx.x * 8 + k] = r[k];
     }
 }
 
 // One block: folds `nblocks` block records (all loaded before they are
-// folded) and writes the box.
+// folded) and writes the box, plus box[4] = the largest |m| (1 if every
+// mass is zero) when `with_mass`.
 __global__ __launch_bounds__(kBoxThreads) void box_finish_kernel(
     const double* __restrict__ partials, unsigned nblocks, double* __restrict__ box,
-    int* __restrict__ err)
+    int* __restrict__ err, bool with_mass)
 {
     constexpr int kPer = kBoxMaxBlocks / kBoxThreads;
-    double rec[kPer][7];
+    double rec[kPer][8];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const unsigned b = threadIdx.x + k * kBoxThreads;
         const double* p = partials + 8 * static_cast<size_t>(b < nblocks ? b : 0);
 #pragma unroll
-        for (int c = 0; c < 7; ++c)
+        for (int c = 0; c < 8; ++c)
             rec[k][c] = p[c];
     }
     const double inf = __builtin_inf();
-    Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false};
+    Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false, 0.0};
     double bad = 0.0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -98,28 +102,30 @@ __global__ __launch_bounds__(kBoxThreads) void box_finish_kernel(
             e.hi[a] = fmax(e.hi[a], rec[k][3 + a]);
         }
         bad = fmax(bad, rec[k][6]);
+        e.mmax = fmax(e.mmax, rec[k][7]);
     }
     e.bad = bad != 0.0;
-    __shared__ double s_red[4][7];
+    __shared__ double s_red[4][8];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    double v[7];
+    double v[8];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         v[a] = wave_min(e.lo[a]);
         v[3 + a] = wave_max(e.hi[a]);
     }
     v[6] = (__ballot(e.bad) != 0ull) ? 1.0 : 0.0;
+    v[7] = wave_max(e.mmax);
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 7; ++k)
+        for (int k = 0; k < 8; ++k)
             s_red[wave][k] = v[k];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double r[7];
+        double r[8];
 #pragma unroll
-        for (int k = 0; k < 7; ++k)
+        for (int k = 0; k < 8; ++k)
             r[k] = s_red[0][k];
         for (int w = 1; w < 4; ++w) {
 #pragma unroll
@@ -128,6 +134,7 @@ __global__ __launch_bounds__(kBoxThreads) void box_finish_kernel(
                 r[3 + a] = fmax(r[3 + a], s_red[w][3 + a]);
             }
             r[6] = fmax(r[6], s_red[w][6]);
+            r[7] = fmax(r[7], s_red[w][7]);
         }
         // globalbounding.cu:122-136
         const double cx = (r[0] + r[3]) * 0.5;
@@ -148,6 +155,8 @@ __global__ __launch_bounds__(kBoxThreads) void box_finish_kernel(
         box[1] = cy;
         box[2] = cz;
         box[3] = side;
+        if (with_mass)
+            box[4] = r[7] > 0.0 ? r[7] : 1.0;
         if (r[6] != 0.0)
             atomicCAS(err, 0, 1);
     }
@@ -159,13 +168,16 @@ __global__ __launch_bounds__(kBoxThreads) void bbox_kernel(
     const double* __restrict__ m, double4* __restrict__ pos4)
 {
     const double inf = __builtin_inf();
-    Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false};
+    Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false, 0.0};
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
          i < n; i += stride) {
         const double px = x[i], py = y[i], pz = z[i];
-        if (pos4)
-            pos4[i] = make_double4(px, py, pz, m[i]);
+        if (pos4) {
+            const double pm = m[i];
+            pos4[i] = make_double4(px, py, pz, pm);
+            e.mmax = fmax(e.mmax, fabs(pm));
+        }
         e.bad = e.bad || !(isfinite(px) && isfinite(py) && isfinite(pz));
         e.lo[0] = fmin(e.lo[0], px);
         e.lo[1] = fmin(e.lo[1], py);
@@ -190,7 +202,7 @@ __global__ __launch_bounds__(kBoxThreads) void kick_drift_bbox_kernel(
 {
     const double half = 0.5 * dt;
     const double inf = __builtin_inf();
-    Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false};
+    Extent e{{inf, inf, inf}, {-inf, -inf, -inf}, false, 0.0};
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
          i < n; i += stride) {
@@ -206,8 +218,11 @@ __global__ __launch_bounds__(kBoxThreads) void kick_drift_bbox_kernel(
         x[i] = px;
         y[i] = py;
         z[i] = pz;
-        if (pos4)
-            pos4[i] = make_double4(px, py, pz, m[i]);
+        if (pos4) {
+            const double pm = m[i];
+            pos4[i] = make_double4(px, py, pz, pm);
+            e.mmax = fmax(e.mmax, fabs(pm));
+        }
         e.bad = e.bad || !(isfinite(px) && isfinite(py) && isfinite(pz));
         e.lo[0] = fmin(e.lo[0], px);
         e.lo[1] = fmin(e.lo[1], py);
@@ -313,14 +328,16 @@ __global__ __launch_bounds__(256) void encode_kernel(
 
 // Morton-order gather (simulation.cu:33-57) into packed FP64 records
 // {x, y, z, m}, plus the fixed-point records of the mixed traversal:
-// coordinates on a 2^31 grid over the box, and G m / unit^2 in FP32 where
-// unit = side / 2^31 (so that a = sum (G m / unit^2) d / r^3 with d, r in
-// grid units is the acceleration in physical units).
+// coordinates on a 2^31 grid over the box, and m / box[4] in FP32, box[4]
+// being the largest |m| (so that sum (m / box[4]) d / r^3, with d and r in
+// grid units, times G box[4] / unit^2, unit = side / 2^31, is the
+// acceleration in physical units; the FP32 sums then stay in range for any
+// unit system).
 __global__ __launch_bounds__(256) void gather_kernel(
     const uint32_t* __restrict__ idx, const double* __restrict__ m,
     const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, size_t n, double4* __restrict__ rec64,
-    int4* __restrict__ rec32, const double* __restrict__ box, double G,
+    int4* __restrict__ rec32, const double* __restrict__ box,
     const double4* __restrict__ pos4)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -353,7 +370,7 @@ __global__ __launch_bounds__(256) void gather_kernel(
         const int qx = static_cast<int>(fmin(fmax(rint((px - ox) * scale), 0.0), lim));
         const int qy = static_cast<int>(fmin(fmax(rint((py - oy) * scale), 0.0), lim));
         const int qz = static_cast<int>(fmin(fmax(rint((pz - oz) * scale), 0.0), lim));
-        const float gm = static_cast<float>(G * pm * scale * scale);
+        const float gm = static_cast<float>(pm / box[4]);
         rec32[i] = make_int4(qx, qy, qz, __float_as_int(gm));
     }
 }
@@ -366,7 +383,8 @@ void launch_bbox(nbh_context* ctx, const double* x, const double* y,
 {
     const unsigned nb = box_blocks(n);
     bbox_kernel<<<nb, kBoxThreads, 0, ctx->stream>>>(x, y, z, n, partials, m, pos4);
-    box_finish_kernel<<<1, kBoxThreads, 0, ctx->stream>>>(partials, nb, d_box, err);
+    box_finish_kernel<<<1, kBoxThreads, 0, ctx->stream>>>(partials, nb, d_box, err,
+                                                         pos4 != nullptr);
     NBH_HIP(hipGetLastError());
 }
 
@@ -380,7 +398,8 @@ void launch_kick_drift_bbox(hipStream_t s, double* x, double* y, double* z,
     const unsigned nb = box_blocks(n);
     kick_drift_bbox_kernel<<<nb, kBoxThreads, 0, s>>>(x, y, z, vx, vy, vz, ax, ay, az, n,
                                                        dt, partials, m, pos4);
-    box_finish_kernel<<<1, kBoxThreads, 0, s>>>(partials, nb, d_box, err);
+    box_finish_kernel<<<1, kBoxThreads, 0, s>>>(partials, nb, d_box, err,
+                                               pos4 != nullptr);
     NBH_HIP(hipGetLastError());
 }
 
@@ -410,10 +429,10 @@ template void launch_encode<21>(hipStream_t, const double*, const double*,
 void launch_gather(hipStream_t s, const uint32_t* idx, const double* m,
                    const double* x, const double* y, const double* z,
                    size_t n, double4* rec64, int4* rec32, const double* box,
-                   double G, const double4* pos4)
+                   const double4* pos4)
 {
     gather_kernel<<<blocks_for(n, 256), 256, 0, s>>>(idx, m, x, y, z, n, rec64,
-                                                     rec32, box, G, pos4);
+                                                     rec32, box, pos4);
     NBH_HIP(hipGetLastError());
 }
 

@@ -608,7 +608,7 @@ int nbh_bh_acceleration(nbh_context* ctx, const nbh_octree_view* octree,
                             particle_order == NBH_ORDER_MORTON, rec, err);
         launch_generic_sizes(s, o, size, local, err);
         launch_walk_build(s, o, true, false, size, local, nullptr, theta, rec,
-                          nullptr, nullptr, gravitational_constant, contains,
+                          nullptr, nullptr, contains,
                           wnode, wcrit, wlink, wcenter, nullptr, nullptr,
                           nullptr, wcount);
         launch_walk_fp64(s, wnode, wcrit, wlink, wcenter, wcount, rec,

@@ -160,8 +160,10 @@ void launch_fill_u64(hipStream_t s, unsigned long long* p,
 
 // bbox_morton.hip
 size_t bbox_partial_doubles();
-// With m and pos4 given, both also write the packed {x, y, z, m} copy of
-// every particle (original order) that the Morton gather reads.
+// d_box receives {centre x, y, z, side}.  With m and pos4 given, both also
+// write the packed {x, y, z, m} copy of every particle (original order) that
+// the Morton gather reads, and d_box[4] = the largest |m| (1 when every mass
+// is zero), the mass unit of the mixed traversal's records.
 void launch_bbox(nbh_context* ctx, const double* x, const double* y,
                  const double* z, size_t n, double* d_box, double* partials,
                  int* err, const double* m = nullptr, double4* pos4 = nullptr);
@@ -181,7 +183,7 @@ void launch_encode(hipStream_t s, const double* x, const double* y,
 void launch_gather(hipStream_t s, const uint32_t* idx, const double* m,
                    const double* x, const double* y, const double* z,
                    size_t n, double4* rec64, int4* rec32, const double* box,
-                   double G, const double4* pos4 = nullptr);
+                   const double4* pos4 = nullptr);
 
 // radix_sort.hip
 size_t sort_status_words(size_t n, int passes);
@@ -239,7 +241,7 @@ void launch_generic_sizes(hipStream_t s, const nbh_octree_view& o, int* size,
 void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
                        bool mixed, const int* size, const int* local,
                        const double* crit, double theta, const double4* rec64,
-                       const int4* rec32, const double* box, double G,
+                       const int4* rec32, const double* box,
                        bool need_center, double4* w64node, double* w64crit,
                        int4* w64link, double4* w64center, int4* w32node,
                        int4* w32link, int4* w32center, int* wcount);
@@ -259,7 +261,8 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx,
                        const uint32_t* order, int n,
-                       float eps2, bool contains, bool soft, double* ax,
+                       float eps2, double out_scale, bool contains, bool soft,
+                       double* ax,
                        double* ay, double* az, int* queue, int* err,
                        unsigned long long* counters);
 void launch_direct(hipStream_t s, const double* m, const double* x,

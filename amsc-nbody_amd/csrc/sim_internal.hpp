@@ -12,7 +12,7 @@ const char* stage_message(int word, int code);
 template <int BITS>
 void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          bool mixed, const double4* rec64, const int4* rec32,
-                         double theta, const double* box, double G,
+                         double theta, const double* box,
                          bool need_center, double eps, int* size, int* local,
                          double* crit, int* pre, int* order, int* level_info,
                          double4* sum4, const int* poffset, const int* pcount,
@@ -83,7 +83,7 @@ struct Readback {
     int nnodes;
     int errors[ERR_WORD_COUNT];
     int pad;
-    double box[4];
+    double box[5]; // centre x, y, z, side, largest |m|
 };
 
 } // namespace nbh
@@ -124,7 +124,7 @@ struct nbh_sim {
     unsigned long long* plan_status = nullptr;
     int* errors = nullptr;
     // small device scalars
-    nbh::DevBuf<double> box;      // 4
+    nbh::DevBuf<double> box;      // centre x, y, z, side, largest |m|
     nbh::DevBuf<double> partials; // bbox partials
     nbh::DevBuf<int> scalars;     // [1] ngroups [2] nnodes [3] wcount ([0] unused)
     nbh::DevBuf<unsigned long long> counters;

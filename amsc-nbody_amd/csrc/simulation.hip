@@ -86,7 +86,7 @@ void alloc_state(nbh_sim* s)
     off += plan_w;
     s->errors = reinterpret_cast<int*>(z + off);
     s->zero_words = off + err_w;
-    s->box.alloc(4);
+    s->box.alloc(5);
     s->partials.alloc(bbox_partial_doubles());
     s->scalars.alloc(16);
     s->counters.alloc(4);
@@ -167,8 +167,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
                            s->idx_alt.p, n, s->passes, s->hist, s->sort_status);
     s->mark(PH_GATHER);
     launch_gather(st, s->idx.p, s->m.p, s->x.p, s->y.p, s->z.p, n, s->rec64.p,
-                  mixed(s) ? s->rec32.p : nullptr, s->box.p,
-                  s->cfg.gravitational_constant, s->pos4.p);
+                  mixed(s) ? s->rec32.p : nullptr, s->box.p, s->pos4.p);
     s->mark(PH_GROUPS);
     int* d_ng = s->scalars.p + 1;
     int* d_nm = s->scalars.p + 2;
@@ -188,7 +187,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
     NBH_HIP(hipMemcpyAsync(s->rb->errors, s->errors,
                            ERR_WORD_COUNT * sizeof(int), hipMemcpyDeviceToHost,
                            st));
-    NBH_HIP(hipMemcpyAsync(s->rb->box, s->box.p, 4 * sizeof(double),
+    NBH_HIP(hipMemcpyAsync(s->rb->box, s->box.p, 5 * sizeof(double),
                            hipMemcpyDeviceToHost, st));
     NBH_HIP(hipStreamSynchronize(st));
     if (s->rb->errors[ERR_BBOX] != 0)
@@ -219,7 +218,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
     const bool contains = s->cfg.theta >= kContainsTheta;
     launch_level_passes<BITS>(
         st, o, Layout<BITS>::bits, mixed(s), s->rec64.p, s->rec32.p,
-        s->cfg.theta, s->box.p, s->cfg.gravitational_constant, contains,
+        s->cfg.theta, s->box.p, contains,
         s->cfg.softening, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->sum4.p, s->poffset.p, s->pcount.p, s->ngroups,
         s->w64node.p, s->w64crit.p,
         s->w64link.p, s->w64center.p, s->w32node.p, nullptr,
@@ -237,11 +236,14 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters)
     if (!counters)
         s->mark(PH_TRAVERSAL);
     if (mixed(s)) {
+        // Records hold m / box[4] on the 2^31 grid of the box: the FP32 sums
+        // times G box[4] scale^2 are the physical accelerations.
         const double scale = 2147483648.0 / s->rb->box[3];
         const double eq = eps * scale;
+        const double out_scale = s->cfg.gravitational_constant * s->rb->box[4] * scale * scale;
         launch_walk_mixed(st, s->w32node.p, s->mcap, s->w32center.p,
                           s->scalars.p + 3, s->rec32.p, s->idx.p, s->horder.p, ni,
-                          static_cast<float>(eq * eq), contains, eps > 0.0,
+                          static_cast<float>(eq * eq), out_scale, contains, eps > 0.0,
                           a[0].p, a[1].p, a[2].p, s->walkq.p,
                           s->errors + ERR_TRAVERSAL, counters);
     } else {

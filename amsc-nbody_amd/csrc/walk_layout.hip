@@ -154,7 +154,7 @@ __device__ __forceinline__ void write_record(
     const nbh_octree_view& o, int v, int pre, const int* __restrict__ size,
     const double* __restrict__ crit_in, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, double G, bool need_center, double eps,
+    const double* __restrict__ box, bool need_center, double eps,
     const WalkOut& w, const double4* __restrict__ sum4, double4 geom)
 {
     double4* w64node = w.w64node;
@@ -244,7 +244,7 @@ __device__ __forceinline__ void write_record(
             const int qx = static_cast<int>(fmin(fmax(rint((cx - ox) * scale), 0.0), lim));
             const int qy = static_cast<int>(fmin(fmax(rint((cy - oy) * scale), 0.0), lim));
             const int qz = static_cast<int>(fmin(fmax(rint((cz - oz) * scale), 0.0), lim));
-            const float gm = static_cast<float>(G * m * scale * scale);
+            const float gm = static_cast<float>(m / box[4]);
             w32node[2 * static_cast<size_t>(pre)] = make_int4(qx, qy, qz, __float_as_int(gm));
         }
         // The traversal compares the softened squared distance d^2 + eps^2
@@ -258,7 +258,11 @@ __device__ __forceinline__ void write_record(
         // particle and a zero mass both add exactly zero.  Records of every
         // other non-internal kind carry crit^2 = +inf, so that the
         // traversal's MAC never accepts them.
-        const bool soft = eps > 0.0;
+        // The always-accepted self-interaction needs inv^3 = (eps^2)^-1.5
+        // to stay finite in FP32 (its product with d = 0 is then exactly
+        // zero); with a smaller grid-unit softening single particles take
+        // the exact slow path, which skips a coincident pair.
+        const bool soft = eq * eq >= 1e-20;
         const bool fast = soft && ((kind == KIND_LEAF && count == 1) || kind == KIND_SKIP);
         const int kind_out = fast ? KIND_INTERNAL : kind;
         if (fast)
@@ -286,7 +290,7 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
     const int* __restrict__ local, const double* __restrict__ crit_in,
     double theta, const double4* __restrict__ rec64,
     const int4* __restrict__ rec32, const double* __restrict__ box,
-    double G, bool need_center, WalkOut w, int* __restrict__ wcount)
+    bool need_center, WalkOut w, int* __restrict__ wcount)
 {
     const int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (v >= o.n_nodes)
@@ -297,7 +301,7 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
     if (v == 0)
         *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
     write_record<GENERIC, MIXED>(o, v, pre, size, crit_in, theta, rec64, rec32,
-                                 box, G, need_center, 0.0, w, nullptr,
+                                 box, need_center, 0.0, w, nullptr,
                                  make_double4(0.0, 0.0, 0.0, 0.0));
 }
 
@@ -447,7 +451,7 @@ __global__ __launch_bounds__(256) void level_down_kernel(
     const int* __restrict__ local, int* __restrict__ pre_of,
     const double* __restrict__ crit, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, double G, bool need_center, double eps,
+    const double* __restrict__ box, bool need_center, double eps,
     WalkOut w, int* __restrict__ wcount, const int* __restrict__ order,
     const int* __restrict__ start, const double4* __restrict__ sum4,
     HilbertOut h)
@@ -488,7 +492,7 @@ __global__ __launch_bounds__(256) void level_down_kernel(
                                 geom.x, geom.y, geom.z, geom.w, code);
         }
         write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32,
-                                   box, G, need_center, eps, w, sum4, geom);
+                                   box, need_center, eps, w, sum4, geom);
     }
 }
 
@@ -684,7 +688,7 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
     const int* __restrict__ local, int* __restrict__ pre_of,
     const double* __restrict__ crit, double theta,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, double G, bool need_center, double eps,
+    const double* __restrict__ box, bool need_center, double eps,
     WalkOut w, const double4* __restrict__ sum4, HilbertOut h)
 {
     using Key = typename Layout<BITS>::Key;
@@ -717,7 +721,7 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
         node_geometry<BITS>(prefix[v], o.level[v], box[0], box[1], box[2], box[3],
                             geom.x, geom.y, geom.z, geom.w, code);
     }
-    write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32, box, G,
+    write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32, box,
                                need_center, eps, w, sum4, geom);
 }
 
@@ -748,33 +752,26 @@ void launch_generic_sizes(hipStream_t s, const nbh_octree_view& o, int* size,
 void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
                        bool mixed, const int* size, const int* local,
                        const double* crit, double theta, const double4* rec64,
-                       const int4* rec32, const double* box, double G,
+                       const int4* rec32, const double* box,
                        bool need_center, double4* w64node, double* w64crit,
                        int4* w64link, double4* w64center, int4* w32node,
                        int4* w32link, int4* w32center, int* wcount)
 {
-    if (!generic)
-        fail(NBH_LOGIC_ERROR, "the per-node walk build serves caller octrees only");
+    if (!generic || mixed)
+        fail(NBH_LOGIC_ERROR,
+             "the per-node walk build serves caller octrees in FP64 only");
+    (void)w32node;
+    (void)w32link;
+    (void)w32center;
     const unsigned blocks = blocks_for(o.n_nodes, 256);
     const int M = o.n_nodes;
-    // Mixed records are interleaved {node, link} pairs (see below).
-    (void)w32link;
-    int4* w32pair_link = w32node ? w32node + 1 : nullptr;
-    if (mixed)
-        link_sentinel_kernel<<<blocks, 256, 0, s>>>(
-            w32pair_link, make_int4(0x7F800000, M, KIND_ERROR + 4, 0), M, 2);
-    else
-        link_sentinel_kernel<<<blocks, 256, 0, s>>>(
-            w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M, 1);
+    link_sentinel_kernel<<<blocks, 256, 0, s>>>(
+        w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M, 1);
     NBH_HIP(hipGetLastError());
-    const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32pair_link,
-                    w32center};
-#define NBH_WALK_ARGS o, size, local, crit, theta, rec64, rec32, box, G, need_center, w, wcount
-    if (!mixed)
-        walk_build_kernel<true, false><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
-    else
-        walk_build_kernel<true, true><<<blocks, 256, 0, s>>>(NBH_WALK_ARGS);
-#undef NBH_WALK_ARGS
+    const WalkOut w{w64node, w64crit, w64link, w64center, nullptr, nullptr,
+                    nullptr};
+    walk_build_kernel<true, false><<<blocks, 256, 0, s>>>(
+        o, size, local, crit, theta, rec64, rec32, box, need_center, w, wcount);
     NBH_HIP(hipGetLastError());
 }
 
@@ -784,7 +781,7 @@ void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
 template <int BITS>
 void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          bool mixed, const double4* rec64, const int4* rec32,
-                         double theta, const double* box, double G,
+                         double theta, const double* box,
                          bool need_center, double eps, int* size, int* local, double* crit,
                          int* pre, int* order, int* level_info, double4* sum4,
                          const int* poffset, const int* pcount, int n_groups,
@@ -842,11 +839,11 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     for (int L = 0; L <= max_level; ++L) {
         if (mixed)
             level_down_kernel<BITS, true><<<blocks, 256, 0, s>>>(
-                o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
+                o, L, size, local, pre, crit, theta, rec64, rec32, box,
                 need_center, eps, w, wcount, order, start, sum4, h);
         else
             level_down_kernel<BITS, false><<<blocks, 256, 0, s>>>(
-                o, L, size, local, pre, crit, theta, rec64, rec32, box, G,
+                o, L, size, local, pre, crit, theta, rec64, rec32, box,
                 need_center, eps, w, wcount, order, start, sum4, h);
         NBH_HIP(hipGetLastError());
     }
@@ -855,11 +852,11 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     if (mixed)
         leaf_down_kernel<BITS, true><<<gblocks, 256, 0, s>>>(
             o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
-            box, G, need_center, eps, w, sum4, h);
+            box, need_center, eps, w, sum4, h);
     else
         leaf_down_kernel<BITS, false><<<gblocks, 256, 0, s>>>(
             o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
-            box, G, need_center, eps, w, sum4, h);
+            box, need_center, eps, w, sum4, h);
     NBH_HIP(hipGetLastError());
 }
 
@@ -898,7 +895,7 @@ template void launch_octree_views<21>(hipStream_t, const nbh_octree_view&,
 #define NBH_LEVEL_INST(B)                                                            \
     template void launch_level_passes<B>(hipStream_t, const nbh_octree_view&, int, bool, \
                                          const double4*, const int4*, double,           \
-                                         const double*, double, bool, double, int*, int*, \
+                                         const double*, bool, double, int*, int*,       \
                                          double*, int*, int*, int*, double4*,           \
                                          const int*, const int*, int, double4*,          \
                                          double*, int4*,                                 \

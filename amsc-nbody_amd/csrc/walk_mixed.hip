@@ -373,8 +373,12 @@ __device__ __forceinline__ void visit_rare(Walker& w, const WalkRec& r,
             separation(s, w.tq, sx, sy, sz, s2);
             const float gm = __int_as_float(s.w);
             bool use = active && k != w.t && gm != 0.0f;
-            if (!SOFT && use && s2 == 0.0f) {
-                w.code = 6;
+            // A coincident pair: an error without softening; with it the
+            // pair adds exactly zero (its direction is zero), which FP32
+            // would lose to inf * 0 when eps^2 is tiny.
+            if (use && s2 == 0.0f) {
+                if (!SOFT)
+                    w.code = 6;
                 use = false;
             }
             interact(sx, sy, sz, s2, gm, eps2, use, w.a.px, w.a.py, w.a.pz);
@@ -426,7 +430,8 @@ __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__
 
 template <bool COUNT>
 __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ idx,
-                                       int n, double* __restrict__ ax_out,
+                                       int n, double out_scale,
+                                       double* __restrict__ ax_out,
                                        double* __restrict__ ay_out,
                                        double* __restrict__ az_out, int* err)
 {
@@ -442,9 +447,9 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
         raise_error(err, 3);
         return;
     }
-    ax_out[o] = w.a.kx.value();
-    ay_out[o] = w.a.ky.value();
-    az_out[o] = w.a.kz.value();
+    ax_out[o] = out_scale * w.a.kx.value();
+    ay_out[o] = out_scale * w.a.ky.value();
+    az_out[o] = out_scale * w.a.kz.value();
 }
 
 // The work queue: packets are dealt from eight ranges, one eighth of the
@@ -497,7 +502,8 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const int4* __restrict__ wcenter, const int* __restrict__ d_wcount,
     const int4* __restrict__ rec, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ order, int n,
-    int* __restrict__ queue, float eps2, double* __restrict__ ax_out,
+    int* __restrict__ queue, float eps2, double out_scale,
+    double* __restrict__ ax_out,
     double* __restrict__ ay_out, double* __restrict__ az_out, int* err,
     unsigned long long* __restrict__ counters)
 {
@@ -565,7 +571,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             visit_rare<SOFT, COUNT>(A, ra, rec, eps2);
             end_visit(A);
         } else {
-            finish<COUNT>(A, idx, n, ax_out, ay_out, az_out, err);
+            finish<COUNT>(A, idx, n, out_scale, ax_out, ay_out, az_out, err);
             const int p = q.take();
             if (p >= 0)
                 A.start(rec, order, p, lane, n);
@@ -583,7 +589,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             visit_rare<SOFT, COUNT>(B, rb, rec, eps2);
             end_visit(B);
         } else {
-            finish<COUNT>(B, idx, n, ax_out, ay_out, az_out, err);
+            finish<COUNT>(B, idx, n, out_scale, ax_out, ay_out, az_out, err);
             const int p = q.take();
             if (p >= 0)
                 B.start(rec, order, p, lane, n);
@@ -593,11 +599,11 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     }
     if (run_a) {
         walk_rest<CONTAINS, SOFT, COUNT>(A, wrec, wcenter, rec, wcount, eps2);
-        finish<COUNT>(A, idx, n, ax_out, ay_out, az_out, err);
+        finish<COUNT>(A, idx, n, out_scale, ax_out, ay_out, az_out, err);
     }
     if (run_b) {
         walk_rest<CONTAINS, SOFT, COUNT>(B, wrec, wcenter, rec, wcount, eps2);
-        finish<COUNT>(B, idx, n, ax_out, ay_out, az_out, err);
+        finish<COUNT>(B, idx, n, out_scale, ax_out, ay_out, az_out, err);
     }
     if (COUNT) {
         unsigned long long npp = A.npp + B.npp, npn = A.npn + B.npn;
@@ -640,8 +646,8 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx,
                        const uint32_t* order, int n,
-                       float eps2, bool contains, bool soft, double* ax,
-                       double* ay, double* az, int* queue, int* err,
+                       float eps2, double out_scale, bool contains, bool soft,
+                       double* ax, double* ay, double* az, int* queue, int* err,
                        unsigned long long* counters)
 {
     // 32-bit record offsets while the record array spans less than 2 GiB
@@ -655,7 +661,7 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
     const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
 #define NBH_ARGS                                                                 \
     wrec, static_cast<unsigned>(wide ? 0 : wbytes), wcenter, d_wcount, rec, idx, \
-        order, n, queue, eps2, ax, ay, az, err, counters
+        order, n, queue, eps2, out_scale, ax, ay, az, err, counters
 #define NBH_LAUNCH(C, S, K)                                                      \
     do {                                                                         \
         if (wide)                                                                \

@@ -80,7 +80,7 @@ def traversal_traffic():
 def energy_of(nb, state, args):
     """(E, K, U) of a state: an FP64 simulation of it, initialised (tree and
     forces), then its FP64 potential walk."""
-    s = nb.Simulation(state, args.dt, args.theta, args.softening, gravitational_constant=1.0,
+    s = nb.Simulation(state, args.dt, args.theta, args.softening, gravitational_constant=args.g,
                       precision=nb.PRECISION_FP64, bits_per_axis=21)
     s.initialize()
     k, u = s.energy()
@@ -94,7 +94,7 @@ def accuracy_at(nb, torch, p, theta, args, targets, exact):
     out = {}
     acc = {}
     for name, prec in (("mixed", nb.PRECISION_MIXED), ("fp64", nb.PRECISION_FP64)):
-        s = nb.Simulation(p, args.dt, theta, args.softening, gravitational_constant=1.0,
+        s = nb.Simulation(p, args.dt, theta, args.softening, gravitational_constant=args.g,
                           precision=prec, bits_per_axis=21)
         s.set_timing(True)
         s.initialize()
@@ -149,7 +149,7 @@ def cpu_baseline(p, args):
     import oracle
     n = len(p["m"])
     sample = min(args.cpu_targets, n)
-    sim = oracle.Sim(p, args.dt, args.theta, args.softening, 1.0, 21)
+    sim = oracle.Sim(p, args.dt, args.theta, args.softening, args.g, 21)
     t0 = time.perf_counter()
     sim.rebuild()
     t_build = time.perf_counter() - t0
@@ -174,6 +174,10 @@ def main():
     ap.add_argument("--softening", type=float, default=1e-3)
     ap.add_argument("--dt", type=float, default=1e-4)
     ap.add_argument("--workload", default="plummer")
+    ap.add_argument("--g", type=float, default=1.0, help="gravitational constant")
+    ap.add_argument("--particle-mass", type=float, default=None,
+                    help="give every particle this mass (the reference's benchmark "
+                         "uses 1e30 with --g 6.6743e-11)")
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--direct-targets", type=int, default=8192)
     ap.add_argument("--cpu-targets", type=int, default=524288)
@@ -205,7 +209,9 @@ def main():
 
     n = args.n
     p = nb.generate(args.workload, n, 42)
-    common = dict(gravitational_constant=1.0, bits_per_axis=21)
+    if args.particle_mass is not None:
+        p["m"][:] = args.particle_mass
+    common = dict(gravitational_constant=args.g, bits_per_axis=21)
 
     mixed = nb.Simulation(p, args.dt, args.theta, args.softening,
                           precision=nb.PRECISION_MIXED, **common)
@@ -231,7 +237,7 @@ def main():
     dev = [torch.from_numpy(p[k]).cuda() for k in ("m", "x", "y", "z")]
     exact = [t.cpu().numpy() for t in
              nb.direct_sum(*dev, torch.from_numpy(targets.view(np.int32)).cuda(),
-                           args.softening, 1.0)]
+                           args.softening, args.g)]
     acc["rms_vs_direct"], acc["max_vs_direct"] = rel_err([a[targets] for a in a_mixed0], exact)
     if not args.no_fp64:
         acc["fp64_rms_vs_direct"], acc["fp64_max_vs_direct"] = rel_err(
@@ -283,7 +289,9 @@ def main():
         "dtype": "f32 traversal / f64 build+integrate",
         "data": "synthetic (seeded generator)",
         "config": {"workload": f"{args.workload} N={n} theta={args.theta} "
-                               f"eps={args.softening} dt={args.dt} G=1 63-bit keys",
+                               f"eps={args.softening} dt={args.dt} G={args.g:g}"
+                               + (f" m={args.particle_mass:g}" if args.particle_mass else "")
+                               + " 63-bit keys",
                    "n_particles": n, "theta": args.theta, "softening": args.softening,
                    "dt": args.dt, "precision": "mixed"},
         "steps_per_s": round(1e3 / ms_per_step, 4),

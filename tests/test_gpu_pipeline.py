@@ -169,6 +169,55 @@ def test_mixed_walk_independent_of_schedule(nb):
         s.close()
 
 
+@pytest.mark.parametrize("kind,pos_scale,mass,G,eps", [
+    ("blob", 1.0, 1e30, 6.6743e-11, 1e-6),    # SI units: G m s^2 beyond FP32
+    ("plummer", 1e15, 1.0, 1.0, 1e-6),        # eps^2 in grid units ~1e-24
+    ("clumps", 1.0, 1e-40, 1.0, 1e-3),        # masses below FP32's range
+    ("uniform", 1e-20, 3.0, 1e5, 1e-23),
+])
+def test_mixed_pipeline_any_units(nb, orc, kind, pos_scale, mass, G, eps):
+    """The mixed records hold masses relative to the largest one, so the
+    traversal's FP32 sums stay in range whatever the unit system."""
+    n = 5000
+    p = make(nb, kind, n, seed=3)
+    for f in ("x", "y", "z"):
+        p[f] = p[f] * pos_scale
+    p["m"][:] = mass * (1.0 + 0.5 * np.random.default_rng(1).random(n))
+    sim = nb.Simulation(p, 1e-3, 0.5, eps, gravitational_constant=G,
+                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    ref = orc.Sim(p, 1e-3, 0.5, eps, G, 21)
+    sim.initialize()
+    ref.initialize()
+    rs = ref.state()
+    a = sim.accelerations()
+    assert all(np.isfinite(c).all() for c in a)
+    rms, mx = rel_err(a, [rs["ax"], rs["ay"], rs["az"]])
+    print(f"mixed vs fp64 oracle {kind} x{pos_scale:g} m={mass:g} G={G:g}: "
+          f"rms={rms:.3e} max={mx:.3e}")
+    assert rms < MIXED_RMS and mx < MIXED_MAX
+    sim.close()
+
+
+def test_mixed_pipeline_coincident_particles(nb, orc):
+    """Coincident particles in one leaf add exactly zero to each other under
+    softening, also when eps^2 in grid units is far below FP32's range."""
+    p = make(nb, "plummer", 2000, seed=5)
+    for f in ("x", "y", "z", "m"):
+        p[f][1000:1010] = p[f][0]
+    for eps in (1e-3, 1e-30):
+        sim = nb.Simulation(p, 1e-3, 0.5, eps, gravitational_constant=1.0,
+                            precision=nb.PRECISION_MIXED, bits_per_axis=21)
+        ref = orc.Sim(p, 1e-3, 0.5, eps, 1.0, 21)
+        sim.initialize()
+        ref.initialize()
+        rs = ref.state()
+        a = sim.accelerations()
+        assert all(np.isfinite(c).all() for c in a), eps
+        rms, mx = rel_err(a, [rs["ax"], rs["ay"], rs["az"]])
+        assert rms < MIXED_RMS and mx < MIXED_MAX, eps
+        sim.close()
+
+
 def test_mixed_pipeline_without_softening(nb, orc):
     p = make(nb, "uniform", 3000)
     sim = nb.Simulation(p, 1e-4, 0.5, 0.0, gravitational_constant=1.0,
