@@ -287,4 +287,15 @@ void launch_pack_records(hipStream_t s, const uint32_t* idx, const double* m,
 // The opening test needs cell geometry only from this theta on.
 constexpr double kContainsTheta = 1.15;
 
+// eps^2 of the mixed traversal in grid units (scale = 2^31 / side), with a
+// floor of 1e-20 under positive softening.  Separations of distinct grid
+// points are at least 1, so the floor changes no interaction's FP32 r^2,
+// while a coincident pair (the target's own record included) keeps a finite
+// inv^3 and so adds exactly zero.
+__host__ __device__ inline double mixed_eps2(double eps, double scale)
+{
+    const double eq = eps * scale;
+    return eps > 0.0 ? fmax(eq * eq, 1e-20) : 0.0;
+}
+
 } // namespace nbh

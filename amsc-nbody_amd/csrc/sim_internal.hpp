@@ -20,8 +20,8 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          int4* w64link, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
                          hipEvent_t between, int* hcount = nullptr,
-                         int* hoffset = nullptr, uint8_t* hstate = nullptr,
-                         uint32_t* horder = nullptr, const uint8_t* htab = nullptr);
+                         int* hoffset = nullptr, uint32_t* horder = nullptr,
+                         const uint8_t* htab = nullptr);
 // ints of the level-pass bookkeeping array (level starts + per-block counts)
 size_t level_info_ints();
 void launch_walk_sentinel(hipStream_t s, int4* w32node, int4* w64link, int cap);
@@ -145,7 +145,6 @@ struct nbh_sim {
     nbh::DevBuf<int4> w64link, w32node, w32center;
     // mixed: the traversal's target order along the Hilbert curve
     nbh::DevBuf<int> hcount, hoffset;    // per node
-    nbh::DevBuf<uint8_t> hstate;         // per node
     nbh::DevBuf<uint32_t> horder;        // per particle
     nbh::DevBuf<uint8_t> htab;           // the curve's state tables
 

@@ -116,7 +116,6 @@ void ensure_octree(nbh_sim* s, size_t m)
     if (mixed(s)) {
         s->hcount.alloc(cap);
         s->hoffset.alloc(cap);
-        s->hstate.alloc(cap);
         s->w32node.alloc(2 * cap); // interleaved {node, link} pairs
         if (s->cfg.theta >= kContainsTheta)
             s->w32center.alloc(cap);
@@ -223,7 +222,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
         s->w64node.p, s->w64crit.p,
         s->w64link.p, s->w64center.p, s->w32node.p, nullptr,
         s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr,
-        s->hcount.p, s->hoffset.p, s->hstate.p, s->horder.p, s->htab.p);
+        s->hcount.p, s->hoffset.p, s->horder.p, s->htab.p);
 }
 
 void forces(nbh_sim* s, int buffer, unsigned long long* counters)
@@ -239,11 +238,11 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters)
         // Records hold m / box[4] on the 2^31 grid of the box: the FP32 sums
         // times G box[4] scale^2 are the physical accelerations.
         const double scale = 2147483648.0 / s->rb->box[3];
-        const double eq = eps * scale;
         const double out_scale = s->cfg.gravitational_constant * s->rb->box[4] * scale * scale;
         launch_walk_mixed(st, s->w32node.p, s->mcap, s->w32center.p,
                           s->scalars.p + 3, s->rec32.p, s->idx.p, s->horder.p, ni,
-                          static_cast<float>(eq * eq), out_scale, contains, eps > 0.0,
+                          static_cast<float>(mixed_eps2(eps, scale)), out_scale, contains,
+                          eps > 0.0,
                           a[0].p, a[1].p, a[2].p, s->walkq.p,
                           s->errors + ERR_TRAVERSAL, counters);
     } else {

@@ -125,14 +125,32 @@ __device__ __forceinline__ int preorder(const nbh_octree_view& o,
 // Outputs of the walk build (records indexed by pre-order position).
 // The mixed traversal's target order along the Hilbert curve (see
 // hilbert.cpp), built alongside the level passes; all pointers are null
-// when the order is not wanted.
+// when the order is not wanted.  A node's curve offset is the rank of its
+// subtree's first particle along the curve: the bottom-up pass writes each
+// child's offset relative to its parent, the top-down pass adds the
+// parent's absolute offset (as for pre-order positions).
 struct HilbertOut {
     int* count;          // subtree particle count per node (bottom-up)
-    int* offset;         // curve rank of the subtree's first particle
-    uint8_t* state;      // curve orientation state per node (top-down)
+    int* offset;         // curve offset, relative to the parent until made absolute
     uint32_t* order;     // curve rank -> sorted particle index
     const uint8_t* tab;  // HilbertTables: rank[24][8] then next[24][8]
 };
+
+constexpr int kHilbertTabBytes = 2 * 8 * kHilbertStates;
+
+// Curve orientation state of the level-L cell with (left-aligned) Morton
+// prefix p: the root's state 0 followed down the prefix's octant digits.
+template <int BITS>
+__device__ __forceinline__ int curve_state(typename Layout<BITS>::Key p, int L,
+                                           const uint8_t* next)
+{
+    int st = 0;
+    for (int d = 1; d <= L; ++d) {
+        const int oct = static_cast<int>((p >> (Layout<BITS>::spatial_bits - 3 * d)) & 0x7u);
+        st = next[8 * st + oct];
+    }
+    return st;
+}
 
 struct WalkOut {
     double4* w64node;
@@ -251,18 +269,14 @@ __device__ __forceinline__ void write_record(
         // (its interaction's r^2) with crit^2 + eps^2, both in fixed-point
         // units.
         const double cq = crit * scale;
-        const double eq = eps * scale;
-        float crit2 = static_cast<float>(fma(cq, cq, eq * eq));
+        const double eq2 = mixed_eps2(eps, scale);
+        float crit2 = static_cast<float>(fma(cq, cq, eq2));
         // With positive softening a single-particle leaf and a zero-mass
         // node are "always accept" monopoles (crit^2 = -1): the target's own
         // particle and a zero mass both add exactly zero.  Records of every
         // other non-internal kind carry crit^2 = +inf, so that the
         // traversal's MAC never accepts them.
-        // The always-accepted self-interaction needs inv^3 = (eps^2)^-1.5
-        // to stay finite in FP32 (its product with d = 0 is then exactly
-        // zero); with a smaller grid-unit softening single particles take
-        // the exact slow path, which skips a coincident pair.
-        const bool soft = eq * eq >= 1e-20;
+        const bool soft = eps > 0.0;
         const bool fast = soft && ((kind == KIND_LEAF && count == 1) || kind == KIND_SKIP);
         const int kind_out = fast ? KIND_INTERNAL : kind;
         if (fast)
@@ -324,6 +338,11 @@ __global__ __launch_bounds__(256) void level_up_kernel(
     double4* __restrict__ sum4, HilbertOut h)
 {
     using Key = typename Layout<BITS>::Key;
+    __shared__ uint8_t s_tab[kHilbertTabBytes];
+    if (h.count)
+        for (int t = static_cast<int>(threadIdx.x); t < kHilbertTabBytes; t += blockDim.x)
+            s_tab[t] = h.tab[t];
+    __syncthreads();
     const double rcx = box[0], rcy = box[1], rcz = box[2], side = box[3];
     const Key* prefix = static_cast<const Key*>(o.prefix);
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
@@ -331,10 +350,10 @@ __global__ __launch_bounds__(256) void level_up_kernel(
     for (int k = start[L] + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
          k < end; k += stride) {
         const int v = order[k];
+        const Key pv = prefix[v];
         double gx, gy, gz, gh;
         int code = 0;
-        node_geometry<BITS>(prefix[v], L, rcx, rcy, rcz, side, gx, gy, gz, gh,
-                            code);
+        node_geometry<BITS>(pv, L, rcx, rcy, rcz, side, gx, gy, gz, gh, code);
         double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
         const int pc = o.particle_count[v];
         int nch = 0, only = 0;
@@ -371,13 +390,29 @@ __global__ __launch_bounds__(256) void level_up_kernel(
             // the node's record on the next top-down level.  Only internal
             // children and single-particle leaves qualify (a leaf holding
             // several particles is summed exactly, never as a monopole).
+            // Curve offsets of the children: in v's orientation state they
+            // take consecutive particle ranges in the order of their ranks.
             if (h.count) {
+                int cnt[8];
                 int hc = 0;
 #pragma unroll
-                for (int oc = 0; oc < 8; ++oc)
-                    if (ch[oc] != -1)
-                        hc += h.count[ch[oc]];
+                for (int oc = 0; oc < 8; ++oc) {
+                    cnt[oc] = ch[oc] != -1 ? h.count[ch[oc]] : 0;
+                    hc += cnt[oc];
+                }
                 h.count[v] = hc;
+                const uint8_t* rank =
+                    s_tab + 8 * curve_state<BITS>(pv, L, s_tab + 8 * kHilbertStates);
+#pragma unroll
+                for (int oc = 0; oc < 8; ++oc) {
+                    if (ch[oc] == -1)
+                        continue;
+                    int before = 0;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                        before += (rank[b] < rank[oc]) ? cnt[b] : 0;
+                    h.offset[ch[oc]] = before;
+                }
             }
             int off = (collapse && nch == 1 && o.particle_count[only] <= 1) ? 0 : 1;
 #pragma unroll
@@ -411,40 +446,6 @@ __global__ __launch_bounds__(256) void level_up_kernel(
 
 // Top-down over one level: pre-order position = parent's + own offset, then
 // the node's walk record.
-// Curve offsets and states of node v's children: the children in curve
-// order (their ranks in v's orientation state) take consecutive particle
-// ranges from v's offset.  The root is in state 0 at offset 0.
-__device__ __forceinline__ void curve_children(const nbh_octree_view& o, int v, int L,
-                                               const HilbertOut& h)
-{
-    const int st = (L == 0) ? 0 : h.state[v];
-    const int off = (L == 0) ? 0 : h.offset[v];
-    if (o.particle_count[v] > 0)
-        return;
-    const int4 c0 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v)];
-    const int4 c1 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v) + 1];
-    const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-    const uint8_t* rank = h.tab + 8 * st;
-    const uint8_t* next = h.tab + 8 * kHilbertStates + 8 * st;
-    int cnt[8], rk[8];
-#pragma unroll
-    for (int oc = 0; oc < 8; ++oc) {
-        cnt[oc] = ch[oc] >= 0 ? h.count[ch[oc]] : 0;
-        rk[oc] = rank[oc];
-    }
-#pragma unroll
-    for (int oc = 0; oc < 8; ++oc) {
-        if (ch[oc] < 0)
-            continue;
-        int before = off;
-#pragma unroll
-        for (int b = 0; b < 8; ++b)
-            before += (rk[b] < rk[oc]) ? cnt[b] : 0;
-        h.offset[ch[oc]] = before;
-        h.state[ch[oc]] = next[oc];
-    }
-}
-
 template <int BITS, bool MIXED>
 __global__ __launch_bounds__(256) void level_down_kernel(
     nbh_octree_view o, int L, const int* __restrict__ size,
@@ -463,10 +464,10 @@ __global__ __launch_bounds__(256) void level_down_kernel(
     for (int k = start[L] + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
          k < end; k += stride) {
         const int v = order[k];
-        if (h.count)
-            curve_children(o, v, L, h);
         int pre = 0;
         if (L == 0) {
+            if (h.count)
+                h.offset[v] = 0;
             const int wc = size[0] < o.n_nodes ? size[0] : o.n_nodes;
             *wcount = wc;
             // Terminal record at the walk length ("error 4, skip to the
@@ -480,8 +481,12 @@ __global__ __launch_bounds__(256) void level_down_kernel(
                     make_int4(0x7F800000, wc, KIND_ERROR + 4, 0);
             else
                 w.w64link[wc] = make_int4(wc, KIND_ERROR + 4, 0, 0);
-        } else
-            pre = pre_of[o.parent[v]] + local[v];
+        } else {
+            const int parent = o.parent[v];
+            pre = pre_of[parent] + local[v];
+            if (h.count)
+                h.offset[v] += h.offset[parent];
+        }
         pre_of[v] = pre;
         if (pre < 0 || pre >= o.n_nodes)
             continue;
@@ -698,17 +703,19 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
     const int v = leaf_of_group(poffset, pcount, o, g);
     if (v < 0)
         return;
+    const int parent = o.parent[v];
+    const bool has_parent = parent >= 0 && parent < o.n_nodes;
     if (h.order) {
-        // the leaf's particles, in sorted order, at its curve offset
-        const int off = (v == 0) ? 0 : h.offset[v];
+        // the leaf's particles, in sorted order, at its curve offset (its
+        // parent's offset is absolute once every internal level is done)
+        const int off = (v == 0) ? 0 : has_parent ? h.offset[v] + h.offset[parent] : -1;
         const int f = o.first_particle[v];
         const int pc = o.particle_count[v];
         if (off >= 0 && pc > 0 && off <= o.n_particles - pc)
             for (int k = 0; k < pc; ++k)
                 h.order[off + k] = static_cast<uint32_t>(f + k);
     }
-    const int parent = o.parent[v];
-    if (parent < 0 || parent >= o.n_nodes)
+    if (!has_parent)
         return;
     const int pre = pre_of[parent] + local[v];
     pre_of[v] = pre;
@@ -789,12 +796,12 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          int4* w64link, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
                          hipEvent_t between, int* hcount, int* hoffset,
-                         uint8_t* hstate, uint32_t* horder, const uint8_t* htab)
+                         uint32_t* horder, const uint8_t* htab)
 {
     // Mixed records are interleaved {node, link} pairs: w32node is the array
     // base and w32link = w32node + 1, both indexed at 2 * position.
     (void)w32link;
-    const HilbertOut h{hcount, hoffset, hstate, horder, htab};
+    const HilbertOut h{hcount, hoffset, horder, htab};
     int4* w32pair_link = w32node ? w32node + 1 : nullptr;
     // Single-child nodes are collapsed into their child in the mixed walk:
     // they carry the child's mass and centre of mass, and for theta below
@@ -900,7 +907,7 @@ template void launch_octree_views<21>(hipStream_t, const nbh_octree_view&,
                                          const int*, const int*, int, double4*,          \
                                          double*, int4*,                                 \
                                          double4*, int4*, int4*, int4*, int*, hipEvent_t, \
-                                         int*, int*, uint8_t*, uint32_t*, const uint8_t*);
+                                         int*, int*, uint32_t*, const uint8_t*);
 NBH_LEVEL_INST(10)
 NBH_LEVEL_INST(21)
 #undef NBH_LEVEL_INST

@@ -109,7 +109,8 @@ __device__ __forceinline__ void interact(float dx, float dy, float dz, float d2,
 
 constexpr int kIdle = 0x7FFFFFFF;
 
-// One packet's walk: 64 Morton-consecutive targets, one per lane.  A lane's
+// One packet's walk: 64 targets consecutive in the target order (the
+// Hilbert curve through the octree), one per lane.  A lane's
 // `next` is the first position it still has to see.  At position i a lane
 // is active when next <= i.  Accepting (or finishing a leaf) sets
 // next = skip; opening leaves next <= i, so the lane is active again at
@@ -373,12 +374,8 @@ __device__ __forceinline__ void visit_rare(Walker& w, const WalkRec& r,
             separation(s, w.tq, sx, sy, sz, s2);
             const float gm = __int_as_float(s.w);
             bool use = active && k != w.t && gm != 0.0f;
-            // A coincident pair: an error without softening; with it the
-            // pair adds exactly zero (its direction is zero), which FP32
-            // would lose to inf * 0 when eps^2 is tiny.
-            if (use && s2 == 0.0f) {
-                if (!SOFT)
-                    w.code = 6;
+            if (!SOFT && use && s2 == 0.0f) {
+                w.code = 6;
                 use = false;
             }
             interact(sx, sy, sz, s2, gm, eps2, use, w.a.px, w.a.py, w.a.pz);
@@ -453,7 +450,7 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
 }
 
 // The work queue: packets are dealt from eight ranges, one eighth of the
-// Morton order each, through one counter per range.  A wavefront takes
+// target order each, through one counter per range.  A wavefront takes
 // packets from the range of its own XCD, so each XCD's L2 serves one region
 // of space, and from the other ranges once its own is empty.
 struct Queue {
