@@ -255,6 +255,15 @@ void launch_walk_fp64(hipStream_t s, const double4* wnode, const double* wcrit,
 // wpairs: interleaved {node, link} int4 pairs, 32 bytes per position.
 // queue: kWalkQueueInts ints of work-queue counters (cleared by the launch).
 constexpr int kWalkQueueInts = 256;
+// Split traversal (few packets): at most kWalkSplitMax subtrees per packet,
+// for walks of at most kWalkSplitPackets packets.  split: 1 + 2 *
+// kWalkSplitMax ints; part: kWalkSplitMax * 64 * 3 doubles per packet.
+constexpr int kWalkSplitMax = 8;
+constexpr int kWalkSplitPackets = 4096; // N <= 262,144
+constexpr int kWalkSplitInts = 1 + 2 * kWalkSplitMax;
+// Whether the mixed traversal of n targets is split (a function of n only,
+// so that results do not depend on the device).
+bool walk_mixed_splits(int n);
 // wcap: records the wpairs buffer holds.
 // order: sorted target indices in the packets' order (null: Morton order).
 void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
@@ -262,8 +271,8 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
                        const int4* rec, const uint32_t* idx,
                        const uint32_t* order, int n,
                        float eps2, double out_scale, bool contains, bool soft,
-                       double* ax,
-                       double* ay, double* az, int* queue, int* err,
+                       double* ax, double* ay, double* az, int* queue,
+                       int* split, double* part, int* err,
                        unsigned long long* counters);
 void launch_direct(hipStream_t s, const double* m, const double* x,
                    const double* y, const double* z, size_t n,

@@ -129,6 +129,10 @@ struct nbh_sim {
     nbh::DevBuf<int> scalars;     // [1] ngroups [2] nnodes [3] wcount ([0] unused)
     nbh::DevBuf<unsigned long long> counters;
     nbh::DevBuf<int> walkq;       // mixed traversal work-queue counters
+    // mixed traversal split into subtree units (few packets only): the
+    // units' position ranges and their partial sums
+    nbh::DevBuf<int> wsplit;
+    nbh::DevBuf<double> wpart;
 
     // octree (capacity grows)
     size_t mcap = 0;

@@ -54,6 +54,10 @@ void alloc_state(nbh_sim* s)
         s->htab.alloc(sizeof(HilbertTables));
         NBH_HIP(hipMemcpyAsync(s->htab.p, &tab, sizeof(HilbertTables),
                                hipMemcpyHostToDevice, s->stream));
+        if (walk_mixed_splits(static_cast<int>(n))) {
+            s->wsplit.alloc(kWalkSplitInts);
+            s->wpart.alloc((n + 63) / 64 * kWalkSplitMax * 64 * 3);
+        }
     }
     s->gfirst.alloc(n);
     s->gcount.alloc(n);
@@ -243,7 +247,7 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters)
                           s->scalars.p + 3, s->rec32.p, s->idx.p, s->horder.p, ni,
                           static_cast<float>(mixed_eps2(eps, scale)), out_scale, contains,
                           eps > 0.0,
-                          a[0].p, a[1].p, a[2].p, s->walkq.p,
+                          a[0].p, a[1].p, a[2].p, s->walkq.p, s->wsplit.p, s->wpart.p,
                           s->errors + ERR_TRAVERSAL, counters);
     } else {
         launch_walk_fp64(st, s->w64node.p, s->w64crit.p, s->w64link.p,

@@ -4,8 +4,16 @@
 // differences of 31-bit fixed-point coordinates over the bounding box, i.e.
 // on coordinates relative to the target: the only rounding of a separation
 // is the conversion of the exact integer difference, relative to the
-// separation itself.  Node records hold G m / unit^2, so the sums are
-// accelerations in physical units.
+// separation itself.  Node records hold m / m_max (m_max the largest |m|),
+// and a target's sums times G m_max / unit^2 are its acceleration in
+// physical units.
+//
+// Units of work.  A unit is a packet of 64 targets and a range of walk
+// positions.  With many packets a unit is a whole walk.  With few packets
+// (small N) the walk is split below its first record, which every target
+// opens: each subtree of that record is a unit of its own, the units write
+// FP64 partial sums, and a reduction adds a packet's partials in subtree
+// order.  Both forms give every target the same interaction list.
 //
 // Hot path.  Internal nodes, single-particle leaves and zero-mass nodes
 // share one straight-line path: "accept if d^2 > crit^2", with crit^2 = -1
@@ -34,8 +42,8 @@ constexpr int kFold = 64;
 // ints between the work-queue counters of two ranges (one per 128 bytes)
 constexpr int kQueueStride = kWalkQueueInts / 8;
 
-// One position of the mixed walk: the node (fixed-point COM, G m) and its
-// link (crit^2, skip, kind | count << 8, first particle).
+// One position of the mixed walk: the node (fixed-point COM, m / m_max) and
+// its link (crit^2 + eps^2, skip, kind | count << 8, first particle).
 struct alignas(32) WalkRec {
     int4 node;
     int4 link;
@@ -124,6 +132,8 @@ struct Walker {
     int next;
     int code;      // traversal error of this lane, 0 if none
     int i;         // current position (wave-uniform)
+    int end;       // end of the unit's positions (split walks only)
+    int unit;      // the unit walked (split walks only)
     unsigned cnt;  // visits since the packet's last fold (wave-uniform)
     Acc a;
     unsigned long long npp, npn, visits;
@@ -346,20 +356,29 @@ __device__ __forceinline__ int skip_vgpr_off(__amdgpu_buffer_rsrc_t rsrc, unsign
 // Next position after an internal-node visit: the active lanes that opened
 // the node are still at or before i (skip > i, so an accepting lane has
 // left).
+// A split unit that leaves its range jumps to the walk length, whose
+// terminal record ends the walk as it ends a whole one.
+template <bool SPLIT>
+__device__ __forceinline__ int next_position(const Walker& w, int c, int wcount)
+{
+    return (SPLIT && c >= w.end) ? wcount : c;
+}
+
+template <bool SPLIT>
 __device__ __forceinline__ void advance_internal(Walker& w, const WalkRec& r,
                                                  unsigned long long actm,
-                                                 unsigned long long accm)
+                                                 unsigned long long accm, int wcount)
 {
-    w.i = (accm != actm) ? w.i + 1 : r.link.y;
+    w.i = next_position<SPLIT>(w, (accm != actm) ? w.i + 1 : r.link.y, wcount);
 }
 
 // A record of another kind below the walk length, after visit_mac: a
 // multi-particle leaf (or a single particle without softening), a
 // zero-mass node to skip, or an error record.
-template <bool SOFT, bool COUNT>
+template <bool SOFT, bool COUNT, bool SPLIT>
 __device__ __forceinline__ void visit_rare(Walker& w, const WalkRec& r,
                                         const int4* __restrict__ rec,
-                                        float eps2)
+                                        float eps2, int wcount)
 {
     const int4 lk = r.link;
     const int skip = lk.y;
@@ -390,12 +409,13 @@ __device__ __forceinline__ void visit_rare(Walker& w, const WalkRec& r,
     }
     if (w.code != 0)
         w.next = kIdle;
-    w.i = (__ballot(w.next <= w.i) != 0ull) ? w.i + 1 : skip;
+    w.i = next_position<SPLIT>(w, (__ballot(w.next <= w.i) != 0ull) ? w.i + 1 : skip,
+                               wcount);
 }
 
 // Walks one packet from its current position to the end.  Position wcount
 // holds a record of a non-internal kind, which ends the internal-node loop.
-template <bool CONTAINS, bool SOFT, bool COUNT>
+template <bool CONTAINS, bool SOFT, bool COUNT, bool SPLIT>
 __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__ wrec,
                                           const int4* __restrict__ wcenter,
                                           const int4* __restrict__ rec,
@@ -413,24 +433,27 @@ __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__
                 ++w.visits;
             ++w.cnt;
             end_visit(w);
-            advance_internal(w, r, actm, accm);
+            advance_internal<SPLIT>(w, r, actm, accm, wcount);
         }
         if (w.i >= wcount)
             break;
         if (COUNT)
             ++w.visits;
-        visit_rare<SOFT, COUNT>(w, r, rec, eps2);
+        visit_rare<SOFT, COUNT, SPLIT>(w, r, rec, eps2, wcount);
         ++w.cnt;
         end_visit(w);
     }
 }
 
-template <bool COUNT>
+// A whole walk writes the target's acceleration; a split unit writes its
+// FP64 partial sums at part[(unit * 64 + lane) * 3].
+template <bool COUNT, bool SPLIT>
 __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ idx,
                                        int n, double out_scale,
                                        double* __restrict__ ax_out,
                                        double* __restrict__ ay_out,
-                                       double* __restrict__ az_out, int* err)
+                                       double* __restrict__ az_out,
+                                       double* __restrict__ part, int* err)
 {
     w.a.fold();
     if (w.code != 0) {
@@ -439,6 +462,13 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
     }
     if (COUNT || !w.valid)
         return;
+    if (SPLIT) {
+        double* q = part + (static_cast<size_t>(w.unit) * 64 + (threadIdx.x & 63)) * 3;
+        q[0] = w.a.kx.value();
+        q[1] = w.a.ky.value();
+        q[2] = w.a.kz.value();
+        return;
+    }
     const uint32_t o = idx[w.t];
     if (o >= static_cast<uint32_t>(n)) {
         raise_error(err, 3);
@@ -485,15 +515,17 @@ __device__ __forceinline__ int xcd_id()
     return static_cast<int>(v & 7u);
 }
 
-// Persistent wavefronts, each with two packet slots walked in lock-step:
-// one iteration loads both slots' records, then does both visits, so the
-// two record loads (mostly served by L2: the walk's working set exceeds the
+// Persistent wavefronts, each with two unit slots walked in lock-step: one
+// iteration loads both slots' records, then does both visits, so the two
+// record loads (mostly served by L2: the walk's working set exceeds the
 // scalar cache) are in flight together and each visit's VALU work overlaps
-// the other slot's load.  A slot whose packet ends writes its results and
-// takes the next packet from the queue, so the slots stay paired until the
+// the other slot's load.  A slot whose unit ends writes its results and
+// takes the next unit from the queue, so the slots stay paired until the
 // queue is empty; the remaining slot then finishes alone.  Every wavefront
 // leaves once the queue is empty and its slots are done.
-template <bool CONTAINS, bool SOFT, bool COUNT, bool WIDE>
+// SPLIT: `split` = {subtrees s, then s {first, end} position pairs}, and
+// unit u walks subtree u % s of packet u / s.
+template <bool CONTAINS, bool SOFT, bool COUNT, bool WIDE, bool SPLIT>
 __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const WalkRec* __restrict__ wrec, unsigned wbytes,
     const int4* __restrict__ wcenter, const int* __restrict__ d_wcount,
@@ -501,25 +533,42 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const uint32_t* __restrict__ order, int n,
     int* __restrict__ queue, float eps2, double out_scale,
     double* __restrict__ ax_out,
-    double* __restrict__ ay_out, double* __restrict__ az_out, int* err,
+    double* __restrict__ ay_out, double* __restrict__ az_out,
+    const int* __restrict__ split, double* __restrict__ part, int* err,
     unsigned long long* __restrict__ counters)
 {
     const int lane = threadIdx.x & 63;
     const int packets = (n + 63) / 64;
-    Queue q{queue, packets, (packets + 7) / 8, xcd_id(), 0};
-    const int pa = q.take();
-    if (pa < 0)
+    const int subtrees = SPLIT ? split[0] : 1;
+    const int units = packets * subtrees;
+    Queue q{queue, units, (units + 7) / 8, xcd_id(), 0};
+    const int ua = q.take();
+    if (ua < 0)
         return;
     const int wcount = *d_wcount;
     const __amdgpu_buffer_rsrc_t rsrc = records_rsrc(wrec, WIDE ? 0u : wbytes);
+    auto begin = [&](Walker& w, int u) {
+        if (SPLIT) {
+            const int p = u / subtrees;
+            const int k = u - p * subtrees;
+            w.start(rec, order, p, lane, n);
+            const int first = split[1 + 2 * k];
+            w.i = first;
+            w.next = w.valid ? first : kIdle;
+            w.end = split[2 + 2 * k];
+            w.unit = u;
+        } else {
+            w.start(rec, order, u, lane, n);
+        }
+    };
     Walker A, B;
     A.npp = A.npn = A.visits = 0;
     B.npp = B.npn = B.visits = 0;
-    A.start(rec, order, pa, lane, n);
-    const int pb = q.take();
-    bool run_a = true, run_b = pb >= 0;
+    begin(A, ua);
+    const int ub = q.take();
+    bool run_a = true, run_b = ub >= 0;
     if (run_b)
-        B.start(rec, order, pb, lane, n);
+        begin(B, ub);
     while (run_a && run_b) {
         // Lock-step iterations until a slot reaches a rare record kind, its
         // end, or its fold point; the loop body has no branch besides its
@@ -552,26 +601,26 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
                 ++A.visits;
                 ++B.visits;
             }
-            advance_internal(A, ra, act_a, acc_a);
-            advance_internal(B, rb, act_b, acc_b);
+            advance_internal<SPLIT>(A, ra, act_a, acc_a, wcount);
+            advance_internal<SPLIT>(B, rb, act_b, acc_b, wcount);
         }
         // Finish both visits of the last iteration; a slot at the end of
-        // its walk moves on to the next packet.
+        // its walk moves on to the next unit.
         if ((ra.link.z & 0xFF) == KIND_INTERNAL) {
             if (COUNT)
                 ++A.visits;
             end_visit(A);
-            advance_internal(A, ra, act_a, acc_a);
+            advance_internal<SPLIT>(A, ra, act_a, acc_a, wcount);
         } else if (A.i < wcount) {
             if (COUNT)
                 ++A.visits;
-            visit_rare<SOFT, COUNT>(A, ra, rec, eps2);
+            visit_rare<SOFT, COUNT, SPLIT>(A, ra, rec, eps2, wcount);
             end_visit(A);
         } else {
-            finish<COUNT>(A, idx, n, out_scale, ax_out, ay_out, az_out, err);
-            const int p = q.take();
-            if (p >= 0)
-                A.start(rec, order, p, lane, n);
+            finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+            const int u = q.take();
+            if (u >= 0)
+                begin(A, u);
             else
                 run_a = false;
         }
@@ -579,28 +628,28 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             if (COUNT)
                 ++B.visits;
             end_visit(B);
-            advance_internal(B, rb, act_b, acc_b);
+            advance_internal<SPLIT>(B, rb, act_b, acc_b, wcount);
         } else if (B.i < wcount) {
             if (COUNT)
                 ++B.visits;
-            visit_rare<SOFT, COUNT>(B, rb, rec, eps2);
+            visit_rare<SOFT, COUNT, SPLIT>(B, rb, rec, eps2, wcount);
             end_visit(B);
         } else {
-            finish<COUNT>(B, idx, n, out_scale, ax_out, ay_out, az_out, err);
-            const int p = q.take();
-            if (p >= 0)
-                B.start(rec, order, p, lane, n);
+            finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+            const int u = q.take();
+            if (u >= 0)
+                begin(B, u);
             else
                 run_b = false;
         }
     }
     if (run_a) {
-        walk_rest<CONTAINS, SOFT, COUNT>(A, wrec, wcenter, rec, wcount, eps2);
-        finish<COUNT>(A, idx, n, out_scale, ax_out, ay_out, az_out, err);
+        walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(A, wrec, wcenter, rec, wcount, eps2);
+        finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
     }
     if (run_b) {
-        walk_rest<CONTAINS, SOFT, COUNT>(B, wrec, wcenter, rec, wcount, eps2);
-        finish<COUNT>(B, idx, n, out_scale, ax_out, ay_out, az_out, err);
+        walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(B, wrec, wcenter, rec, wcount, eps2);
+        finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
     }
     if (COUNT) {
         unsigned long long npp = A.npp + B.npp, npn = A.npn + B.npn;
@@ -616,6 +665,79 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     }
 }
 
+// The split of the walk below its first record: the subtrees that follow
+// position 0, read off the skip links.  Every target opens position 0 (a
+// cell holding the target is never accepted: for theta < 2/sqrt(3) its
+// opening radius exceeds every distance inside it, and above that the
+// containment test rejects it), so each subtree is walked by every lane
+// from its first position.  Without such a record (a one-record walk, or
+// links that do not tile the walk) the single unit is the whole walk.
+// split = {s, first_0, end_0, ..., first_{s-1}, end_{s-1}}.
+__global__ void walk_split_kernel(const WalkRec* __restrict__ wrec,
+                                  const int* __restrict__ d_wcount,
+                                  int* __restrict__ split)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const int wc = *d_wcount;
+    bool ok = wc > 1 && (wrec[0].link.z & 0xFF) == KIND_INTERNAL &&
+              __int_as_float(wrec[0].link.x) >= 0.0f;
+    int k = 0;
+    int p = 1;
+    while (ok && p < wc) {
+        const int e = wrec[p].link.y;
+        if (k == kWalkSplitMax || e <= p || e > wc) {
+            ok = false;
+            break;
+        }
+        split[1 + 2 * k] = p;
+        split[2 + 2 * k] = e;
+        p = e;
+        ++k;
+    }
+    if (!ok || k == 0) {
+        k = 1;
+        split[1] = 0;
+        split[2] = wc;
+    }
+    split[0] = k;
+}
+
+// Target accelerations of a split walk: each target's unit partials added
+// in subtree order, scaled to physical units and written to the target's
+// original index.
+__global__ __launch_bounds__(256) void walk_split_sum_kernel(
+    const double* __restrict__ part, const int* __restrict__ split,
+    const uint32_t* __restrict__ order, const uint32_t* __restrict__ idx, int n,
+    double out_scale, double* __restrict__ ax_out, double* __restrict__ ay_out,
+    double* __restrict__ az_out)
+{
+    const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n)
+        return;
+    const int subtrees = split[0];
+    const int packet = k >> 6;
+    const int lane = k & 63;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int j = 0; j < subtrees; ++j) {
+        const double* q =
+            part + ((static_cast<size_t>(packet) * subtrees + j) * 64 + lane) * 3;
+        sx += q[0];
+        sy += q[1];
+        sz += q[2];
+    }
+    // out-of-range entries were reported by the walk (error 3)
+    const uint32_t t = order ? order[k] : static_cast<uint32_t>(k);
+    if (t >= static_cast<uint32_t>(n))
+        return;
+    const uint32_t o = idx[t];
+    if (o >= static_cast<uint32_t>(n))
+        return;
+    ax_out[o] = out_scale * sx;
+    ay_out[o] = out_scale * sy;
+    az_out[o] = out_scale * sz;
+}
+
 __global__ void queue_clear_kernel(int* __restrict__ queue)
 {
     if (threadIdx.x < 8)
@@ -625,8 +747,8 @@ __global__ void queue_clear_kernel(int* __restrict__ queue)
 } // namespace
 
 // One wavefront per resident slot: 32 per CU (8 per SIMD), fewer when
-// there are fewer packet pairs.
-static int walk_persistent_blocks(int n)
+// there are fewer pairs of units.
+static int walk_persistent_blocks(long long units)
 {
     static int cus[64] = {};
     int dev = 0;
@@ -634,9 +756,14 @@ static int walk_persistent_blocks(int n)
     int& c = cus[dev & 63];
     if (c == 0)
         NBH_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
-    const int pairs = ((n + 63) / 64 + 1) / 2;
-    const int waves = std::min(pairs, 32 * c);
+    const long long pairs = (units + 1) / 2;
+    const int waves = static_cast<int>(std::min<long long>(pairs, 32LL * c));
     return (waves + 3) / 4;
+}
+
+bool walk_mixed_splits(int n)
+{
+    return n > 0 && (n + 63) / 64 <= kWalkSplitPackets;
 }
 
 void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
@@ -644,27 +771,38 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
                        const int4* rec, const uint32_t* idx,
                        const uint32_t* order, int n,
                        float eps2, double out_scale, bool contains, bool soft,
-                       double* ax, double* ay, double* az, int* queue, int* err,
+                       double* ax, double* ay, double* az, int* queue,
+                       int* split, double* part, int* err,
                        unsigned long long* counters)
 {
     // 32-bit record offsets while the record array spans less than 2 GiB
     const size_t wbytes = wcap * sizeof(WalkRec);
     const bool wide = wbytes > 0x7FFFFFFFull;
+    const bool splits = split != nullptr;
+    if (splits && (wide || !part || !walk_mixed_splits(n)))
+        fail(NBH_LOGIC_ERROR, "a split traversal needs fewer packets and a partial-sum buffer");
     queue_clear_kernel<<<1, 64, 0, s>>>(queue);
     NBH_HIP(hipGetLastError());
-    const int nb = walk_persistent_blocks(n);
+    const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
+    if (splits) {
+        walk_split_kernel<<<1, 64, 0, s>>>(wrec, d_wcount, split);
+        NBH_HIP(hipGetLastError());
+    }
+    const long long packets = (static_cast<long long>(n) + 63) / 64;
+    const int nb = walk_persistent_blocks(splits ? packets * kWalkSplitMax : packets);
     if (nb == 0)
         return;
-    const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
 #define NBH_ARGS                                                                 \
     wrec, static_cast<unsigned>(wide ? 0 : wbytes), wcenter, d_wcount, rec, idx, \
-        order, n, queue, eps2, out_scale, ax, ay, az, err, counters
+        order, n, queue, eps2, out_scale, ax, ay, az, split, part, err, counters
 #define NBH_LAUNCH(C, S, K)                                                      \
     do {                                                                         \
-        if (wide)                                                                \
-            walk_mixed_kernel<C, S, K, true><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS); \
+        if (splits)                                                              \
+            walk_mixed_kernel<C, S, K, false, true><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS); \
+        else if (wide)                                                           \
+            walk_mixed_kernel<C, S, K, true, false><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS); \
         else                                                                     \
-            walk_mixed_kernel<C, S, K, false><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS); \
+            walk_mixed_kernel<C, S, K, false, false><<<nb, kWalkThreads, 0, s>>>(NBH_ARGS); \
     } while (0)
     const bool cnt = counters != nullptr;
     if (contains) {
@@ -683,6 +821,11 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
 #undef NBH_LAUNCH
 #undef NBH_ARGS
     NBH_HIP(hipGetLastError());
+    if (splits && !cnt) {
+        walk_split_sum_kernel<<<blocks_for(static_cast<size_t>(n), 256), 256, 0, s>>>(
+            part, split, order, idx, n, out_scale, ax, ay, az);
+        NBH_HIP(hipGetLastError());
+    }
 }
 
 } // namespace nbh

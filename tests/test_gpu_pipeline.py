@@ -144,11 +144,14 @@ def test_mixed_walk_single_particle_is_zero(nb):
         assert np.array_equal(a, np.zeros(1))
 
 
-def test_mixed_walk_independent_of_schedule(nb):
-    """Wavefronts take packets from a queue in whatever order they run; each
-    packet's sums depend only on its targets, so two simulations of the same
-    input, and the same state walked again, agree bit for bit."""
-    p = make(nb, "clumps", 50_000, seed=3)
+@pytest.mark.parametrize("n", [50_000, 300_000])
+def test_mixed_walk_independent_of_schedule(nb, n):
+    """Wavefronts take units of work from a queue in whatever order they run;
+    each unit's sums depend only on its targets and positions, so two
+    simulations of the same input, and the same state walked again, agree
+    bit for bit.  50K targets split each packet's walk into subtree units;
+    300K walk whole packets."""
+    p = make(nb, "clumps", n, seed=3)
     sims = [nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
                           precision=nb.PRECISION_MIXED, bits_per_axis=21)
             for _ in range(2)]
@@ -196,6 +199,25 @@ def test_mixed_pipeline_any_units(nb, orc, kind, pos_scale, mass, G, eps):
           f"rms={rms:.3e} max={mx:.3e}")
     assert rms < MIXED_RMS and mx < MIXED_MAX
     sim.close()
+
+
+@pytest.mark.parametrize("kind,n", [("plummer", 262_144), ("plummer", 262_208),
+                                    ("uniform", 300_000)])
+def test_mixed_split_and_whole_walks_match_fp64(nb, kind, n):
+    """Either side of the size where the walk stops being split into subtree
+    units (4096 packets): the mixed accelerations stay within the mixed
+    tolerance of the FP64 pipeline, which is bit-exact with the oracle."""
+    p = make(nb, kind, n, seed=9)
+    out = {}
+    for prec in (nb.PRECISION_MIXED, nb.PRECISION_FP64):
+        sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
+                            precision=prec, bits_per_axis=21)
+        sim.initialize()
+        out[prec] = [np.asarray(c).copy() for c in sim.accelerations()]
+        sim.close()
+    rms, mx = rel_err(out[nb.PRECISION_MIXED], out[nb.PRECISION_FP64])
+    print(f"mixed vs fp64 pipeline {kind} n={n}: rms={rms:.3e} max={mx:.3e}")
+    assert rms < MIXED_RMS and mx < MIXED_MAX
 
 
 def test_mixed_pipeline_coincident_particles(nb, orc):
