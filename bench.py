@@ -164,6 +164,98 @@ def cpu_baseline(p, args):
                       f"Morton-ordered targets ({t_forces:.2f} s), scaled to N"}
 
 
+def adjoint_workload(args):
+    """The differentiable workload: K leapfrog steps of the FP64 pipeline on
+    the disk + bulge galaxy, reverse mode with checkpoints, and the gradient
+    of L = sum_i |r_i(T) - r*_i|^2 (r* a perturbed copy of the final
+    positions) with respect to the initial velocities and the masses,
+    checked by central differences on a few coordinates."""
+    import torch
+    import nbody_amd as nb
+
+    n, K, every = args.n, args.adjoint_steps, args.adjoint_every
+    p = nb.generate("disk", n, 42)
+    adj = nb.Adjoint(n, args.dt, args.theta, args.softening, K, every,
+                     gravitational_constant=args.g)
+    state = [torch.from_numpy(p[k]).cuda() for k in ("x", "y", "z", "vx", "vy", "vz")]
+    m = torch.from_numpy(p["m"]).cuda()
+
+    def final(mm, st):
+        return np.stack([t.cpu().numpy() for t in adj.forward(mm, *st)[:3]])
+
+    r_end = final(m, state)
+    target = r_end + 1e-3 * np.random.default_rng(1).normal(size=r_end.shape)
+
+    def loss(mm, st):
+        # L(run) - L(base run), from the small difference of the final
+        # positions: sum d (d + 2 (r_end - r*)), d = r(T) - r_end
+        d = final(mm, st) - r_end
+        return float((d * (d + 2.0 * (r_end - target))).sum())
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = adj.forward(m, *state)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    cot = [torch.from_numpy(2.0 * (o.cpu().numpy() - target[k])).cuda()
+           for k, o in enumerate(out[:3])]
+    zeros = [torch.zeros_like(state[0]) for _ in range(3)]
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    grads = adj.backward(*cot, *zeros, with_mass=True)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    gv = np.stack([g.cpu().numpy() for g in grads[3:6]])
+    gm = grads[6].cpu().numpy()
+
+    rng = np.random.default_rng(2)
+    probes = []
+    for k in range(args.adjoint_probes):
+        i = int(rng.integers(n))
+        on_mass = k % 2 == 1
+        c = int(rng.integers(3))
+        h = 1e-5 * p["m"][i] if on_mass else 1e-7
+
+        def f(d):
+            if on_mass:
+                mm = m.clone()
+                mm[i] += d
+                return loss(mm, state)
+            st = [t.clone() for t in state]
+            st[3 + c][i] += d
+            return loss(m, st)
+        fd1 = (f(h) - f(-h)) / (2 * h)
+        fd2 = (f(2 * h) - f(-2 * h)) / (4 * h)
+        ad = float(gm[i] if on_mass else gv[c, i])
+        probes.append({"what": "mass" if on_mass else f"v{'xyz'[c]}", "index": i,
+                       "adjoint": ad, "fd": fd1,
+                       "smooth": bool(abs(fd1 - fd2) <= 1e-4 * abs(fd1)),
+                       "rel_err": abs(ad - fd1) / max(abs(fd1), 1e-300)})
+    def worst(kind):
+        e = [q["rel_err"] for q in probes if q["smooth"] and (q["what"] == "mass") == kind]
+        return (len(e), max(e) if e else None)
+    fwd_ms, bwd_ms = 1e3 * (t1 - t0), 1e3 * (t3 - t2)
+    line = {
+        "metric": "adjoint Mparticle-steps/s (forward + reverse), disk + bulge",
+        "value": round(n * K / ((t1 - t0) + (t3 - t2)) / 1e6, 3),
+        "unit": "Mparticle-steps/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (seeded generator)",
+        "config": {"workload": f"disk+bulge N={n} theta={args.theta} eps={args.softening} "
+                               f"dt={args.dt} K={K} checkpoints every {every}",
+                   "n_particles": n, "steps": K, "checkpoint_every": every},
+        "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
+        "backward_over_forward": round(bwd_ms / fwd_ms, 3),
+        "loss": "sum |r(T) - r*|^2, r* = r(T) + 1e-3 N(0,1)",
+        "fd_probes": probes,
+        # probes whose steps h and 2h agree to 1e-4 (no opening decision
+        # flipped within 2h), and their largest relative error
+        "fd_velocity_smooth_max_rel_err": worst(False),
+        "fd_mass_smooth_max_rel_err": worst(True),
+    }
+    adj.close()
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,11 +282,22 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-energy", action="store_true",
                     help="skip the FP64 energy check (it builds a second, FP64 simulation)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="measure the differentiable workload instead (FP64 forward + "
+                         "reverse over --adjoint-steps steps; use with --n 262144 "
+                         "--theta 0.5 --softening 0.01 --dt 1e-3)")
+    ap.add_argument("--adjoint-steps", type=int, default=100)
+    ap.add_argument("--adjoint-every", type=int, default=10)
+    ap.add_argument("--adjoint-probes", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.adjoint:
+        adjoint_workload(args)
+        return
 
     import torch
     import nbody_amd as nb
