@@ -256,11 +256,11 @@ void launch_walk_fp64(hipStream_t s, const double4* wnode, const double* wcrit,
 // queue: kWalkQueueInts ints of work-queue counters (cleared by the launch).
 constexpr int kWalkQueueInts = 256;
 // Split traversal (few packets): at most kWalkSplitMax subtrees per packet,
-// for walks of at most kWalkSplitPackets packets.  split: 1 + 2 *
+// for walks of at most kWalkSplitPackets packets.  split: 1 + 4 *
 // kWalkSplitMax ints; part: kWalkSplitMax * 64 * 3 doubles per packet.
 constexpr int kWalkSplitMax = 8;
 constexpr int kWalkSplitPackets = 4096; // N <= 262,144
-constexpr int kWalkSplitInts = 1 + 2 * kWalkSplitMax;
+constexpr int kWalkSplitInts = 1 + 4 * kWalkSplitMax;
 // Whether the mixed traversal of n targets is split (a function of n only,
 // so that results do not depend on the device).
 bool walk_mixed_splits(int n);

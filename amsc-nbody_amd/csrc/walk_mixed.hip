@@ -13,7 +13,11 @@
 // (small N) the walk is split below its first record, which every target
 // opens: each subtree of that record is a unit of its own, the units write
 // FP64 partial sums, and a reduction adds a packet's partials in subtree
-// order.  Both forms give every target the same interaction list.
+// order.  Both forms give every target the same interaction list.  The
+// subtrees' first records are marked as boundaries (kind flag, crit^2 =
+// +inf, their own link kept aside), so the straight-line loop leaves at
+// them as at any rare record: a unit ends at the next boundary, and a walk
+// passing one visits it with the kept link.
 //
 // Hot path.  Internal nodes, single-particle leaves and zero-mass nodes
 // share one straight-line path: "accept if d^2 > crit^2", with crit^2 = -1
@@ -39,6 +43,8 @@ namespace nbh {
 namespace {
 
 constexpr int kFold = 64;
+// Kind-byte flag of the records that start a split walk's subtrees.
+constexpr int kBoundary = 0x80;
 // ints between the work-queue counters of two ranges (one per 128 bytes)
 constexpr int kQueueStride = kWalkQueueInts / 8;
 
@@ -132,8 +138,8 @@ struct Walker {
     int next;
     int code;      // traversal error of this lane, 0 if none
     int i;         // current position (wave-uniform)
-    int end;       // end of the unit's positions (split walks only)
-    int unit;      // the unit walked (split walks only)
+    int end;       // end of the unit's positions (the walk length if whole)
+    int unit;      // the unit walked (split walks: its partial-sum slot)
     unsigned cnt;  // visits since the packet's last fold (wave-uniform)
     Acc a;
     unsigned long long npp, npn, visits;
@@ -356,29 +362,20 @@ __device__ __forceinline__ int skip_vgpr_off(__amdgpu_buffer_rsrc_t rsrc, unsign
 // Next position after an internal-node visit: the active lanes that opened
 // the node are still at or before i (skip > i, so an accepting lane has
 // left).
-// A split unit that leaves its range jumps to the walk length, whose
-// terminal record ends the walk as it ends a whole one.
-template <bool SPLIT>
-__device__ __forceinline__ int next_position(const Walker& w, int c, int wcount)
-{
-    return (SPLIT && c >= w.end) ? wcount : c;
-}
-
-template <bool SPLIT>
 __device__ __forceinline__ void advance_internal(Walker& w, const WalkRec& r,
                                                  unsigned long long actm,
-                                                 unsigned long long accm, int wcount)
+                                                 unsigned long long accm)
 {
-    w.i = next_position<SPLIT>(w, (accm != actm) ? w.i + 1 : r.link.y, wcount);
+    w.i = (accm != actm) ? w.i + 1 : r.link.y;
 }
 
 // A record of another kind below the walk length, after visit_mac: a
 // multi-particle leaf (or a single particle without softening), a
 // zero-mass node to skip, or an error record.
-template <bool SOFT, bool COUNT, bool SPLIT>
+template <bool SOFT, bool COUNT>
 __device__ __forceinline__ void visit_rare(Walker& w, const WalkRec& r,
                                         const int4* __restrict__ rec,
-                                        float eps2, int wcount)
+                                        float eps2)
 {
     const int4 lk = r.link;
     const int skip = lk.y;
@@ -409,8 +406,40 @@ __device__ __forceinline__ void visit_rare(Walker& w, const WalkRec& r,
     }
     if (w.code != 0)
         w.next = kIdle;
-    w.i = next_position<SPLIT>(w, (__ballot(w.next <= w.i) != 0ull) ? w.i + 1 : skip,
-                               wcount);
+    w.i = (__ballot(w.next <= w.i) != 0ull) ? w.i + 1 : skip;
+}
+
+// The visit of a boundary record at w.i that does not end the unit: the
+// record with its own link (kept in the split table) takes the usual path.
+// The straight-line loop has already evaluated the marked record, whose
+// crit^2 = +inf accepts nothing and changes no lane.
+template <bool CONTAINS, bool SOFT, bool COUNT>
+__device__ __forceinline__ void visit_boundary(Walker& w, const WalkRec& r,
+                                               const int* __restrict__ split,
+                                               const int4* __restrict__ wcenter,
+                                               const int4* __restrict__ rec,
+                                               float eps2)
+{
+    const int subtrees = split[0];
+    int k = 0;
+    while (k < subtrees && split[1 + 4 * k] != w.i)
+        ++k;
+    if (k == subtrees) { // not a subtree's first record: a corrupt table
+        w.code = 7;
+        w.next = kIdle;
+        w.i = r.link.y;
+        return;
+    }
+    WalkRec own = r;
+    own.link.x = split[3 + 4 * k];
+    own.link.z = split[4 + 4 * k];
+    if ((own.link.z & 0xFF) == KIND_INTERNAL) {
+        unsigned long long actm, accm;
+        visit_mac<CONTAINS, COUNT>(w, own, own.link.y, wcenter, eps2, actm, accm);
+        advance_internal(w, own, actm, accm);
+    } else {
+        visit_rare<SOFT, COUNT>(w, own, rec, eps2);
+    }
 }
 
 // Walks one packet from its current position to the end.  Position wcount
@@ -419,6 +448,7 @@ template <bool CONTAINS, bool SOFT, bool COUNT, bool SPLIT>
 __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__ wrec,
                                           const int4* __restrict__ wcenter,
                                           const int4* __restrict__ rec,
+                                          const int* __restrict__ split,
                                           int wcount, float eps2)
 {
     for (;;) {
@@ -433,21 +463,31 @@ __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__
                 ++w.visits;
             ++w.cnt;
             end_visit(w);
-            advance_internal<SPLIT>(w, r, actm, accm, wcount);
+            advance_internal(w, r, actm, accm);
+        }
+        if (SPLIT && (r.link.z & kBoundary) != 0) {
+            if (w.i == w.end)
+                break;
+            if (COUNT)
+                ++w.visits;
+            visit_boundary<CONTAINS, SOFT, COUNT>(w, r, split, wcenter, rec, eps2);
+            ++w.cnt;
+            end_visit(w);
+            continue;
         }
         if (w.i >= wcount)
             break;
         if (COUNT)
             ++w.visits;
-        visit_rare<SOFT, COUNT, SPLIT>(w, r, rec, eps2, wcount);
+        visit_rare<SOFT, COUNT>(w, r, rec, eps2);
         ++w.cnt;
         end_visit(w);
     }
 }
 
-// A whole walk writes the target's acceleration; a split unit writes its
-// FP64 partial sums at part[(unit * 64 + lane) * 3].
-template <bool COUNT, bool SPLIT>
+// A whole walk writes the target's acceleration; a split unit (part given)
+// writes its FP64 partial sums at part[(unit * 64 + lane) * 3].
+template <bool COUNT>
 __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ idx,
                                        int n, double out_scale,
                                        double* __restrict__ ax_out,
@@ -462,7 +502,7 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
     }
     if (COUNT || !w.valid)
         return;
-    if (SPLIT) {
+    if (part) {
         double* q = part + (static_cast<size_t>(w.unit) * 64 + (threadIdx.x & 63)) * 3;
         q[0] = w.a.kx.value();
         q[1] = w.a.ky.value();
@@ -523,8 +563,9 @@ __device__ __forceinline__ int xcd_id()
 // takes the next unit from the queue, so the slots stay paired until the
 // queue is empty; the remaining slot then finishes alone.  Every wavefront
 // leaves once the queue is empty and its slots are done.
-// SPLIT: `split` = {subtrees s, then s {first, end} position pairs}, and
-// unit u walks subtree u % s of packet u / s.
+// SPLIT (split and part given): unit u walks subtree u % s of packet u / s,
+// with split = {s, then per subtree {first, end, own crit^2, own kind
+// word}}; only these instantiations handle boundary records.
 template <bool CONTAINS, bool SOFT, bool COUNT, bool WIDE, bool SPLIT>
 __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const WalkRec* __restrict__ wrec, unsigned wbytes,
@@ -552,14 +593,15 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             const int p = u / subtrees;
             const int k = u - p * subtrees;
             w.start(rec, order, p, lane, n);
-            const int first = split[1 + 2 * k];
+            const int first = split[1 + 4 * k];
             w.i = first;
             w.next = w.valid ? first : kIdle;
-            w.end = split[2 + 2 * k];
-            w.unit = u;
+            w.end = split[2 + 4 * k];
         } else {
             w.start(rec, order, u, lane, n);
+            w.end = wcount;
         }
+        w.unit = u;
     };
     Walker A, B;
     A.npp = A.npn = A.visits = 0;
@@ -601,8 +643,8 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
                 ++A.visits;
                 ++B.visits;
             }
-            advance_internal<SPLIT>(A, ra, act_a, acc_a, wcount);
-            advance_internal<SPLIT>(B, rb, act_b, acc_b, wcount);
+            advance_internal(A, ra, act_a, acc_a);
+            advance_internal(B, rb, act_b, acc_b);
         }
         // Finish both visits of the last iteration; a slot at the end of
         // its walk moves on to the next unit.
@@ -610,14 +652,19 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             if (COUNT)
                 ++A.visits;
             end_visit(A);
-            advance_internal<SPLIT>(A, ra, act_a, acc_a, wcount);
-        } else if (A.i < wcount) {
+            advance_internal(A, ra, act_a, acc_a);
+        } else if (SPLIT && A.i != A.end && (ra.link.z & kBoundary) != 0) {
             if (COUNT)
                 ++A.visits;
-            visit_rare<SOFT, COUNT, SPLIT>(A, ra, rec, eps2, wcount);
+            visit_boundary<CONTAINS, SOFT, COUNT>(A, ra, split, wcenter, rec, eps2);
+            end_visit(A);
+        } else if (A.i < A.end) {
+            if (COUNT)
+                ++A.visits;
+            visit_rare<SOFT, COUNT>(A, ra, rec, eps2);
             end_visit(A);
         } else {
-            finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+            finish<COUNT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
             const int u = q.take();
             if (u >= 0)
                 begin(A, u);
@@ -628,14 +675,19 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             if (COUNT)
                 ++B.visits;
             end_visit(B);
-            advance_internal<SPLIT>(B, rb, act_b, acc_b, wcount);
-        } else if (B.i < wcount) {
+            advance_internal(B, rb, act_b, acc_b);
+        } else if (SPLIT && B.i != B.end && (rb.link.z & kBoundary) != 0) {
             if (COUNT)
                 ++B.visits;
-            visit_rare<SOFT, COUNT, SPLIT>(B, rb, rec, eps2, wcount);
+            visit_boundary<CONTAINS, SOFT, COUNT>(B, rb, split, wcenter, rec, eps2);
+            end_visit(B);
+        } else if (B.i < B.end) {
+            if (COUNT)
+                ++B.visits;
+            visit_rare<SOFT, COUNT>(B, rb, rec, eps2);
             end_visit(B);
         } else {
-            finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+            finish<COUNT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
             const int u = q.take();
             if (u >= 0)
                 begin(B, u);
@@ -644,12 +696,12 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
         }
     }
     if (run_a) {
-        walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(A, wrec, wcenter, rec, wcount, eps2);
-        finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+        walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(A, wrec, wcenter, rec, split, wcount, eps2);
+        finish<COUNT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
     }
     if (run_b) {
-        walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(B, wrec, wcenter, rec, wcount, eps2);
-        finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+        walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(B, wrec, wcenter, rec, split, wcount, eps2);
+        finish<COUNT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
     }
     if (COUNT) {
         unsigned long long npp = A.npp + B.npp, npn = A.npn + B.npn;
@@ -670,10 +722,15 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
 // cell holding the target is never accepted: for theta < 2/sqrt(3) its
 // opening radius exceeds every distance inside it, and above that the
 // containment test rejects it), so each subtree is walked by every lane
-// from its first position.  Without such a record (a one-record walk, or
-// links that do not tile the walk) the single unit is the whole walk.
-// split = {s, first_0, end_0, ..., first_{s-1}, end_{s-1}}.
-__global__ void walk_split_kernel(const WalkRec* __restrict__ wrec,
+// from its first position.  The subtrees' first records are marked as
+// boundaries (kBoundary in the kind byte, crit^2 = +inf) and their own
+// crit^2 and kind word kept in the table; a record that is already marked
+// was marked by an earlier split of the same layout, whose table entry
+// holds its own values.  Without such a split (a one-record walk, or links
+// that do not tile the walk) the single unit is the whole walk and no
+// record is marked.
+// split = {s, then per subtree {first, end, own crit^2, own kind word}}.
+__global__ void walk_split_kernel(WalkRec* __restrict__ wrec,
                                   const int* __restrict__ d_wcount,
                                   int* __restrict__ split)
 {
@@ -690,15 +747,28 @@ __global__ void walk_split_kernel(const WalkRec* __restrict__ wrec,
             ok = false;
             break;
         }
-        split[1 + 2 * k] = p;
-        split[2 + 2 * k] = e;
         p = e;
         ++k;
     }
     if (!ok || k == 0) {
-        k = 1;
+        split[0] = 1;
         split[1] = 0;
         split[2] = wc;
+        return;
+    }
+    p = 1;
+    for (int j = 0; j < k; ++j) {
+        int4& lk = wrec[p].link;
+        const int e = lk.y;
+        if ((lk.z & kBoundary) == 0) {
+            split[3 + 4 * j] = lk.x;
+            split[4 + 4 * j] = lk.z;
+            lk.x = 0x7F800000;
+            lk.z |= kBoundary;
+        }
+        split[1 + 4 * j] = p;
+        split[2 + 4 * j] = e;
+        p = e;
     }
     split[0] = k;
 }
@@ -785,7 +855,7 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
     NBH_HIP(hipGetLastError());
     const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
     if (splits) {
-        walk_split_kernel<<<1, 64, 0, s>>>(wrec, d_wcount, split);
+        walk_split_kernel<<<1, 64, 0, s>>>(const_cast<WalkRec*>(wrec), d_wcount, split);
         NBH_HIP(hipGetLastError());
     }
     const long long packets = (static_cast<long long>(n) + 63) / 64;
@@ -794,7 +864,8 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
         return;
 #define NBH_ARGS                                                                 \
     wrec, static_cast<unsigned>(wide ? 0 : wbytes), wcenter, d_wcount, rec, idx, \
-        order, n, queue, eps2, out_scale, ax, ay, az, split, part, err, counters
+        order, n, queue, eps2, out_scale, ax, ay, az, splits ? split : nullptr, \
+        splits ? part : nullptr, err, counters
 #define NBH_LAUNCH(C, S, K)                                                      \
     do {                                                                         \
         if (splits)                                                              \
