@@ -266,7 +266,9 @@ constexpr int kWalkSplitInts = 1 + 4 * kWalkSplitMax;
 bool walk_mixed_splits(int n);
 // wcap: records the wpairs buffer holds.
 // order: sorted target indices in the packets' order (null: Morton order).
-void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
+// wpairs: the {node, link} record pairs; a split walk marks its subtrees'
+// first records in place (see walk_split_kernel).
+void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx,
                        const uint32_t* order, int n,

@@ -424,8 +424,8 @@ __device__ __forceinline__ void visit_boundary(Walker& w, const WalkRec& r,
     int k = 0;
     while (k < subtrees && split[1 + 4 * k] != w.i)
         ++k;
-    if (k == subtrees) { // not a subtree's first record: a corrupt table
-        w.code = 7;
+    if (k == subtrees) { // not a subtree's first record: inconsistent walk
+        w.code = 4;
         w.next = kIdle;
         w.i = r.link.y;
         return;
@@ -836,7 +836,7 @@ bool walk_mixed_splits(int n)
     return n > 0 && (n + 63) / 64 <= kWalkSplitPackets;
 }
 
-void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
+void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx,
                        const uint32_t* order, int n,
@@ -853,9 +853,9 @@ void launch_walk_mixed(hipStream_t s, const int4* wpairs, size_t wcap,
         fail(NBH_LOGIC_ERROR, "a split traversal needs fewer packets and a partial-sum buffer");
     queue_clear_kernel<<<1, 64, 0, s>>>(queue);
     NBH_HIP(hipGetLastError());
-    const auto* wrec = reinterpret_cast<const WalkRec*>(wpairs);
+    auto* wrec = reinterpret_cast<WalkRec*>(wpairs);
     if (splits) {
-        walk_split_kernel<<<1, 64, 0, s>>>(const_cast<WalkRec*>(wrec), d_wcount, split);
+        walk_split_kernel<<<1, 64, 0, s>>>(wrec, d_wcount, split);
         NBH_HIP(hipGetLastError());
     }
     const long long packets = (static_cast<long long>(n) + 63) / 64;
