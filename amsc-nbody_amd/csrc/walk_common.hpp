@@ -18,7 +18,8 @@ enum : int {
 
 constexpr int kWalkThreads = 256; // 4 independent wavefronts per block
 
-// Packets are Morton-consecutive; blocks are dealt to the 8 XCDs round
+// Static grids (the FP64 walks): packets are Morton-consecutive; blocks are
+// dealt to the 8 XCDs round
 // robin, so block b takes packet range (b % 8) * per + b / 8: each XCD's L2
 // then serves one contiguous eighth of space.
 __device__ __forceinline__ int packet_of_block(int nblocks_padded)

@@ -8,7 +8,8 @@
 // subtree.  "Open" is then pre + 1 and "done with this subtree" is skip: no
 // stack, no parent reads.
 //
-// Packets.  One wavefront owns 64 Morton-consecutive targets.  The wave
+// Packets.  One wavefront owns 64 targets (Morton-consecutive in the FP64
+// walk, consecutive along a Hilbert curve in the mixed one).  The wave
 // walks the layout in increasing position.  A lane is active at position i
 // when its own next position is <= i; active lanes evaluate the node exactly
 // as the per-target walk would (leaf -> exact particles, zero mass -> skip,
