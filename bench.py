@@ -161,7 +161,44 @@ def cpu_baseline(p, args):
     return {"value": n / step / 1e6, "unit": "Mparticle-steps/s",
             "cores": oracle.threads(), "kind": "port",
             "sample": f"one rebuild at N={n} ({t_build:.2f} s) + forces of {sample} "
-                      f"Morton-ordered targets ({t_forces:.2f} s), scaled to N"}
+                      f"Morton-ordered targets ({t_forces:.2f} s), scaled to N",
+            "small_plummer_end_to_end": small_plummer_end_to_end()}
+
+
+def small_plummer_end_to_end():
+    """The plain CPU case end to end: Plummer N=16384, G=1, theta=0.5,
+    eps=0.01, dt=1e-3, 10 KDK steps, 30-bit keys, on the oracle and on the
+    GPU pipeline (whose FP64 states must equal the oracle's bit for bit)."""
+    import oracle
+    import nbody_amd as nb
+    n, steps = 16384, 10
+    p = nb.generate("plummer", n, 42)
+    ref = oracle.Sim(p, 1e-3, 0.5, 0.01, 1.0, 10)
+    ref.initialize()
+    t0 = time.perf_counter()
+    ref.step(steps)
+    t_cpu = time.perf_counter() - t0
+    out = {"n": n, "steps": steps, "cpu_ms_per_step": round(1e3 * t_cpu / steps, 3)}
+    rs = ref.state()
+    for name, prec in (("fp64", nb.PRECISION_FP64), ("mixed", nb.PRECISION_MIXED)):
+        sim = nb.Simulation(p, 1e-3, 0.5, 0.01, gravitational_constant=1.0,
+                            precision=prec, bits_per_axis=10)
+        sim.initialize()
+        import torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run(steps)
+        torch.cuda.synchronize()
+        out[f"gpu_{name}_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / steps, 3)
+        st = sim.state()
+        if prec == nb.PRECISION_FP64:
+            out["fp64_state_bit_exact"] = all(np.array_equal(st[f], rs[f])
+                                             for f in ("x", "y", "z", "vx", "vy", "vz"))
+        else:
+            dx = np.sqrt(sum((st[f] - rs[f]) ** 2 for f in ("x", "y", "z")))
+            out["mixed_max_position_diff"] = float(dx.max())
+        sim.close()
+    return out
 
 
 def adjoint_workload(args):
