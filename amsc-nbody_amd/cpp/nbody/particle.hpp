@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <fstream>
 #include <stdexcept>
+#include <utility>
 #include <vector>
 
 #include "device_buffer.hpp"
@@ -84,6 +85,14 @@ public:
     {
         return {count_, mass_.data(), x_.data(), y_.data(), z_.data(),
                 vx_.data(), vy_.data(), vz_.data(), type_.data()};
+    }
+
+    // Not in the reference: hands the type codes over to a new owner (the
+    // Simulation, which copies the FP64 state into its own arrays and keeps
+    // only the types of the set it was built from).
+    [[nodiscard]] DeviceBuffer<std::uint8_t> release_types() noexcept
+    {
+        return std::move(type_);
     }
 
 private:

@@ -28,10 +28,12 @@ struct SimulationConfig {
 
 class Simulation {
 public:
-    Simulation(Particles particles, SimulationConfig config)
-        : config_(config), particles_(std::move(particles))
+    // Copies the FP64 state of the moved-in set into the simulation's own
+    // device arrays and frees it, keeping only the type codes: the
+    // simulation holds one device copy of the state.
+    Simulation(Particles particles, SimulationConfig config) : config_(config)
     {
-        const ConstDeviceParticlesView p = std::as_const(particles_).device_view();
+        const ConstDeviceParticlesView p = std::as_const(particles).device_view();
         nbh_sim_config c{};
         c.time_step = config.timeStep;
         c.theta = config.theta;
@@ -41,6 +43,7 @@ public:
         c.bits_per_axis = config.bitsPerAxis;
         nbody_amd::check(nbh_sim_create_device(&sim_, &c, p.count, p.mass, p.x, p.y, p.z,
                                                p.vx, p.vy, p.vz, nullptr));
+        types_ = particles.release_types();
     }
 
     ~Simulation() noexcept
@@ -63,8 +66,7 @@ public:
     {
         nbh_particles_view v{};
         (void)nbh_sim_particles(sim_, &v);
-        return {v.count, v.mass, v.x, v.y, v.z, v.vx, v.vy, v.vz,
-                particles_.device_view().type};
+        return {v.count, v.mass, v.x, v.y, v.z, v.vx, v.vy, v.vz, types_.data()};
     }
 
     // Not in the reference: the underlying C ABI handle (phase timings,
@@ -73,6 +75,6 @@ public:
 
 private:
     SimulationConfig config_;
-    Particles particles_; // keeps the type codes; the state lives in sim_
+    DeviceBuffer<std::uint8_t> types_; // the type codes; the state lives in sim_
     nbh_sim* sim_ = nullptr;
 };

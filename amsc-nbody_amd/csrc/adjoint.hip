@@ -151,6 +151,9 @@ struct nbh_adjoint {
     int every = 1;
     int segments = 0;
     bool forward_done = false;
+    // Count of forward calls: the checkpoints, masses and tree a backward
+    // replays belong to the most recent forward only.
+    uint64_t generation = 0;
     nbh::DevBuf<double> ck;   // segments x 6n: x | y | z | vx | vy | vz
     nbh::DevBuf<double> seg;  // (every + 1) x 3n positions of one segment
     nbh::DevBuf<double> xb, vb, ab; // 3n each
@@ -329,6 +332,7 @@ int nbh_adjoint_forward(nbh_adjoint* a, const double* d_mass, const double* d_x,
         const size_t n = a->n;
         const hipStream_t st = s->stream;
         a->forward_done = false;
+        ++a->generation;
         copy(st, s->m.p, d_mass, n);
         copy(st, s->x.p, d_x, n);
         copy(st, s->y.p, d_y, n);
@@ -350,6 +354,15 @@ int nbh_adjoint_forward(nbh_adjoint* a, const double* d_mass, const double* d_x,
         copy(st, d_vz_out, s->vz.p, n);
         NBH_HIP(hipStreamSynchronize(st));
         a->forward_done = true;
+    });
+}
+
+int nbh_adjoint_generation(const nbh_adjoint* a, uint64_t* generation)
+{
+    return guard([&] {
+        if (!a || !generation)
+            fail(NBH_INVALID_ARGUMENT, "arguments must not be null");
+        *generation = a->generation;
     });
 }
 

@@ -94,6 +94,8 @@ const char* traversal_message(int code)
     case 6: return "Singular gravitational interaction; use positive softening";
     case 7: return "Occupied octree leaf unexpectedly has children";
     case 8: return "Octree node has invalid geometry";
+    case 9: return "Octree leaf holds 2^23 or more particles on one Morton key, more than "
+                   "the mixed-precision walk records can count; use the FP64 pipeline";
     default: return "Unknown Barnes-Hut traversal error";
     }
 }

@@ -16,6 +16,12 @@ enum : int {
     KIND_ERROR = 100     // kind - 100 is the traversal error code
 };
 
+// Mixed walk records keep a leaf's particle count in the 23 bits above the
+// kind byte of the link word; a leaf of kMixedLeafCountLimit or more
+// particles is written as an error record of this traversal code.
+constexpr int kMixedLeafCountLimit = 1 << 23;
+constexpr int kErrMixedLeafTooLarge = 9;
+
 constexpr int kWalkThreads = 256; // 4 independent wavefronts per block
 
 // Static grids (the FP64 walks): packets are Morton-consecutive; blocks are

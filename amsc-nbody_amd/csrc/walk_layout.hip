@@ -279,10 +279,17 @@ __device__ __forceinline__ void write_record(
         // traversal's MAC never accepts them.
         const bool soft = eps > 0.0;
         const bool fast = soft && ((kind == KIND_LEAF && count == 1) || kind == KIND_SKIP);
-        const int kind_out = fast ? KIND_INTERNAL : kind;
+        int kind_out = fast ? KIND_INTERNAL : kind;
+        // The link word holds the leaf's particle count above its kind byte,
+        // 23 bits: a larger leaf (that many particles on one grid key)
+        // becomes an error record, which every walk reaching it reports.
+        if (kind == KIND_LEAF && count >= kMixedLeafCountLimit) {
+            kind_out = KIND_ERROR + kErrMixedLeafTooLarge;
+            count = 0;
+        }
         if (fast)
             crit2 = -1.0f;
-        else if (kind != KIND_INTERNAL)
+        else if (kind_out != KIND_INTERNAL)
             crit2 = __int_as_float(0x7F800000);
         w32link[2 * static_cast<size_t>(pre)] = make_int4(__float_as_int(crit2), skip,
                                  kind_out | (count << 8), first);

@@ -192,6 +192,7 @@ _sig("nbh_adjoint_create", _i, C.POINTER(_p), C.POINTER(SimConfig), _sz, _i, _i,
 _sig("nbh_adjoint_destroy", _i, _p)
 _sig("nbh_adjoint_forward", _i, _p, *([_p] * 13))
 _sig("nbh_adjoint_backward", _i, _p, *([_p] * 13))
+_sig("nbh_adjoint_generation", _i, _p, _p)
 
 
 class FrameHeader(C.Structure):
@@ -691,6 +692,27 @@ def direct_sum(m, x, y, z, targets, softening, G=1.0, ctx=None):
 
 
 # ----------------------------------------------------------- particle files
+def _particle_arrays(particles: dict, keys=("m", "x", "y", "z", "vx", "vy", "vz")):
+    """The named arrays of a particle set as contiguous host arrays (float64,
+    uint8 for "type"), all one-dimensional and of one length; InvalidArgument
+    otherwise, before any pointer reaches the library."""
+    out = []
+    for k in keys:
+        if k not in particles:
+            raise InvalidArgument(f"particle set has no '{k}' array")
+        a = np.asarray(particles[k])
+        if a.ndim != 1:
+            raise InvalidArgument(f"particle array '{k}' must be one-dimensional "
+                                  f"(shape {a.shape})")
+        out.append(np.ascontiguousarray(a, dtype=np.uint8 if k == "type" else np.float64))
+    n = len(out[0])
+    for k, a in zip(keys, out):
+        if len(a) != n:
+            raise InvalidArgument(f"particle array '{k}' has {len(a)} elements, "
+                                  f"'{keys[0]}' has {n}")
+    return out
+
+
 def load_particles(path) -> dict:
     """Reads a planar binary particle file (the reference's input format) into
     host arrays m, x, y, z, vx, vy, vz (float64) and type (uint8)."""
@@ -710,7 +732,8 @@ def load_particles(path) -> dict:
 
 def save_particles(path, particles: dict, wide_header=True, with_types=False):
     """Writes host arrays in the planar binary particle format."""
-    n = len(particles["m"])
+    keys = ("m", "x", "y", "z", "vx", "vy", "vz") + (("type",) if with_types else ())
+    n = len(_particle_arrays(particles, keys)[0])
     with open(path, "wb") as f:
         f.write(np.array([n], dtype=np.uint64 if wide_header else np.uint32).tobytes())
         for k in ("m", "x", "y", "z", "vx", "vy", "vz"):
@@ -908,14 +931,13 @@ class Simulation:
     def __init__(self, particles: dict, time_step: float, theta: float, softening: float,
                  gravitational_constant: float = DEFAULT_G, precision=PRECISION_FP64,
                  bits_per_axis=21, stream=None):
+        self.handle = _p()
+        arr = _particle_arrays(particles)
+        n = len(arr[0])
         device()
         cfg = SimConfig(time_step, theta, softening, gravitational_constant, precision,
                         bits_per_axis)
         self.config = cfg
-        n = len(particles["m"])
-        arr = [np.ascontiguousarray(particles[k], dtype=np.float64)
-               for k in ("m", "x", "y", "z", "vx", "vy", "vz")]
-        self.handle = _p()
         check(lib.nbh_sim_create(C.byref(self.handle), C.byref(cfg), n,
                                  *[a.ctypes.data for a in arr], stream))
         self.count = n
@@ -1085,6 +1107,12 @@ class Adjoint:
                                       *[ptr(t) for t in outs]))
         return outs
 
+    def generation(self) -> int:
+        """Number of forward calls so far; a backward replays the most recent."""
+        g = _u64(0)
+        check(lib.nbh_adjoint_generation(self.handle, C.byref(g)))
+        return g.value
+
     def backward(self, xb, yb, zb, vxb, vyb, vzb, with_mass=False):
         """Cotangents of (x0, y0, z0, vx0, vy0, vz0), and of the masses as a
         seventh output with with_mass."""
@@ -1109,11 +1137,19 @@ def leapfrog_function():
         def forward(ctx, adjoint, m, pos, vel):
             out = adjoint.forward(m, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2])
             ctx.adjoint = adjoint
+            ctx.generation = adjoint.generation()
             return torch.stack(out[:3]), torch.stack(out[3:])
 
         @staticmethod
         def backward(ctx, gpos, gvel):
             torch_ = _torch()
+            # The adjoint replays the checkpoints of its most recent forward:
+            # a later forward on the same object has overwritten this one's.
+            if ctx.adjoint.generation() != ctx.generation:
+                raise RuntimeError(
+                    "LeapfrogBH.backward: the Adjoint ran another forward after this "
+                    "output was computed; run forward again before backward, or use "
+                    "one Adjoint per trajectory")
             if gpos is None:
                 gpos = torch_.zeros(3, ctx.adjoint.count, dtype=torch_.float64, device=device())
             if gvel is None:

@@ -348,6 +348,12 @@ int nbh_adjoint_forward(nbh_adjoint* adjoint, const double* d_mass,
                         const double* d_vz, double* d_x_out, double* d_y_out,
                         double* d_z_out, double* d_vx_out, double* d_vy_out,
                         double* d_vz_out);
+/* Number of nbh_adjoint_forward calls made on `adjoint` so far.  A backward
+ * replays the checkpoints and masses of the most recent forward only, so a
+ * caller holding the result of an earlier forward compares the generation it
+ * recorded after that forward with this one before calling backward.  No
+ * reference counterpart (the reference has no adjoint). */
+int nbh_adjoint_generation(const nbh_adjoint* adjoint, uint64_t* generation);
 int nbh_adjoint_backward(nbh_adjoint* adjoint, const double* d_xbar,
                          const double* d_ybar, const double* d_zbar,
                          const double* d_vxbar, const double* d_vybar,

@@ -345,6 +345,30 @@ def test_autograd_function_matches_adjoint(nb):
     assert rel(m.grad.cpu().numpy(), ref[6].cpu().numpy()) < 1e-12
 
 
+def test_autograd_backward_of_a_superseded_forward_raises(nb):
+    # The adjoint keeps the checkpoints of its latest forward only: the
+    # backward of an earlier output must refuse rather than differentiate
+    # the later trajectory.
+    n, K = 128, 3
+    p = nb.generate("plummer", n, 12)
+    pos, vel = planar(p)
+    adj = nb.Adjoint(n, 1e-3, 0.5, 0.01, K, 1, gravitational_constant=1.0)
+    fn = nb.leapfrog_function()
+    dev = torch.device("cuda", 0)
+    m = torch.tensor(p["m"], device=dev)
+    x0 = torch.tensor(pos, device=dev, requires_grad=True)
+    v0 = torch.tensor(vel, device=dev, requires_grad=True)
+    g0 = adj.generation()
+    xk1, _ = fn.apply(adj, m, x0, v0)
+    xk2, _ = fn.apply(adj, m, x0 * 1.001, v0)
+    assert adj.generation() == g0 + 2
+    with pytest.raises(RuntimeError, match="another forward"):
+        xk1.sum().backward()
+    # the latest output still differentiates
+    xk2.sum().backward()
+    assert torch.isfinite(x0.grad).all()
+
+
 # ------------------------------------------------------------- two bodies
 def _two_body(nb, steps_per_period, periods=1.0):
     # m1 = m2 = 1/2 at separation 1, G = 1: circular, relative speed 1,

@@ -51,3 +51,25 @@ def test_rejections(nb, tmp_path):
         nb.load_particles(f)
     with pytest.raises(nb.DeviceError, match="Could not open particle input file"):
         nb.load_particles(tmp_path / "missing.bin")
+
+
+def test_mismatched_array_lengths_rejected_before_the_library(nb, tmp_path):
+    # A short array would otherwise let the C side read past a host buffer.
+    p = sample(nb)
+    p["vy"] = p["vy"][:-1]
+    with pytest.raises(nb.InvalidArgument, match="'vy' has 256"):
+        nb.save_particles(tmp_path / "p.bin", p)
+    with pytest.raises(nb.InvalidArgument, match="'vy' has 256"):
+        nb.Simulation(p, 1e-3, 0.5, 0.01)
+    q = sample(nb)
+    q["x"] = np.stack([q["x"], q["x"]])
+    with pytest.raises(nb.InvalidArgument, match="one-dimensional"):
+        nb.Simulation(q, 1e-3, 0.5, 0.01)
+    r = sample(nb)
+    del r["vz"]
+    with pytest.raises(nb.InvalidArgument, match="no 'vz'"):
+        nb.save_particles(tmp_path / "p.bin", r)
+    s = sample(nb)
+    s["type"] = s["type"][:10]
+    with pytest.raises(nb.InvalidArgument, match="'type' has 10"):
+        nb.save_particles(tmp_path / "p.bin", s, with_types=True)
