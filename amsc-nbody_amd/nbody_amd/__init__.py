@@ -150,6 +150,7 @@ _sig("nbh_bbox_compute", _i, _p, _p, _p, _p, _sz, _p, C.POINTER(_d))
 _sig("nbh_morton_keys", _i, _p, _p, _p, _p, _sz, _p, _i, _p, _p)
 _sig("nbh_radix_sort_pairs", _i, _p, _p, _p, _sz, _i, _i)
 _sig("nbh_leaf_groups_build", _i, _p, _p, _i, _i, _i, _p, _p, _p, C.POINTER(_i))
+_sig("nbh_leaf_groups_build_noalloc", _i, _p, _p, _i, _i, _i, _p, _p, _p, _p, C.POINTER(_i))
 _sig("nbh_tree_build", _i, _p, C.POINTER(TreeView), _p, _i, _i)
 _sig("nbh_tree_center_of_mass", _i, _p, C.POINTER(TreeView), _p, _p, _p, _p, _p, _p, _p)
 _sig("nbh_radix_to_octree_plan", _i, _p, C.POINTER(TreeView), _p, _i, _p, _p, _p,
@@ -413,6 +414,29 @@ def build_morton_leaf_groups(groups: MortonLeafGroups, sorted_keys, n_particles,
                                     n_particles, groups.capacity, ptr(groups.unique_keys),
                                     ptr(groups.first_particle), ptr(groups.particle_count),
                                     C.byref(ng)))
+    groups.n_particles = n_particles
+    groups.n_groups = ng.value
+
+
+LEAF_GROUPS_NOALLOC_MAX = 1 << 28
+
+
+def build_morton_leaf_groups_noalloc(groups: MortonLeafGroups, sorted_keys, n_particles,
+                                     d_n_groups=None, stream=None, read_count=True):
+    """buildMortonLeafGroups without allocation (no reference counterpart):
+    the scan's scratch lives in a static device array, for at most
+    LEAF_GROUPS_NOALLOC_MAX particles.  d_n_groups: optional device int32
+    tensor receiving the group count; read_count=False returns without a
+    host synchronisation (n_groups is then left at 0)."""
+    groups.n_particles = 0
+    groups.n_groups = 0
+    ng = _i(0)
+    check(lib.nbh_leaf_groups_build_noalloc(stream, ptr(sorted_keys), groups.key_bytes,
+                                            n_particles, groups.capacity,
+                                            ptr(groups.unique_keys), ptr(groups.first_particle),
+                                            ptr(groups.particle_count),
+                                            ptr(d_n_groups) if d_n_groups is not None else None,
+                                            C.byref(ng) if read_count else None))
     groups.n_particles = n_particles
     groups.n_groups = ng.value
 

@@ -96,6 +96,28 @@ int nbh_leaf_groups_build(nbh_context* ctx, const void* d_sorted_keys,
                           void* d_unique_keys, int* d_first_particle,
                           int* d_particle_count, int* h_n_groups);
 
+/* The same build without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * scan's scratch words (look-back status words, tile ticket, and the group
+ * count when d_n_groups is NULL) live in a file-scope device array sized for
+ * NBH_LEAF_GROUPS_NOALLOC_MAX particles (about 1 MiB, held on every device
+ * the library runs on until the process ends); larger n_particles ->
+ * NBH_INVALID_ARGUMENT.  Work is ordered on hip_stream (NULL: the null
+ * stream) with that stream's device current; the call makes no hipMalloc,
+ * hipFree or memset, and clears the status words with a kernel.  The group
+ * count goes to d_n_groups (device int) when given; with h_n_groups the call
+ * also synchronises the stream, validates the count and returns it there,
+ * without it the call returns as soon as the work is queued.  Every call on
+ * a device shares that device's array, which keeps the last call's
+ * contents: calls that overlap on one device race. */
+enum { NBH_LEAF_GROUPS_NOALLOC_MAX = 1 << 28 };
+int nbh_leaf_groups_build_noalloc(void* hip_stream, const void* d_sorted_keys,
+                                  int key_bytes, int n_particles, int capacity,
+                                  void* d_unique_keys, int* d_first_particle,
+                                  int* d_particle_count, int* d_n_groups,
+                                  int* h_n_groups);
+
 /* ------------------------------------------------------- Karras radix tree */
 typedef struct nbh_tree_view {
     int n_leaves;
