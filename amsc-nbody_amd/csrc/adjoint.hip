@@ -26,8 +26,7 @@
 
 namespace nbh {
 
-void launch_potential_fp64(hipStream_t s, const double4* wnode,
-                           const double* wcrit, const int4* wlink,
+void launch_potential_fp64(hipStream_t s, const Walk64Rec* w64,
                            const double4* wcenter, const int* d_wcount,
                            const double4* rec, const uint32_t* idx, int n,
                            double eps2, double G, bool contains, double* phi,
@@ -38,8 +37,7 @@ void launch_energy_partials(hipStream_t s, const double* m, const double* vx,
                             int blocks);
 void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
                            int max_level, const int* pre_of,
-                           const double4* wnode, const double* wcrit,
-                           const int4* wlink, const double4* wcenter,
+                           const Walk64Rec* w64, const double4* wcenter,
                            const int* d_wcount, int wcount_host,
                            const double4* rec, const uint32_t* idx, int n,
                            double eps2, double G, bool contains,
@@ -123,8 +121,7 @@ void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scr
     sim_fill_views(s); // the push-down reads node masses
     nbh_octree_view o = s->octree_view();
     launch_force_vjp_fp64(s->stream, o, s->cfg.bits_per_axis, s->pre.p,
-                          s->w64node.p, s->w64crit.p, s->w64link.p,
-                          s->w64center.p, s->scalars.p + 3, static_cast<int>(M),
+                          s->w64.p, s->w64center.p, s->scalars.p + 3, static_cast<int>(M),
                           s->rec64.p, s->idx.p, static_cast<int>(n), eps * eps,
                           s->cfg.gravitational_constant, contains, abar,
                           abar + n, abar + 2 * n, scratch.p, xbar, xbar + n,
@@ -213,8 +210,8 @@ int nbh_sim_energy(nbh_sim* s, double* kinetic, double* potential)
         partials.alloc(2 * blocks);
         const bool contains = s->cfg.theta >= kContainsTheta;
         const double eps = s->cfg.softening;
-        launch_potential_fp64(s->stream, s->w64node.p, s->w64crit.p, s->w64link.p,
-                              s->w64center.p, s->scalars.p + 3, s->rec64.p,
+        launch_potential_fp64(s->stream, s->w64.p, s->w64center.p, s->scalars.p + 3,
+                              s->rec64.p,
                               s->idx.p, static_cast<int>(n), eps * eps,
                               s->cfg.gravitational_constant, contains, phi.p,
                               s->errors + ERR_TRAVERSAL);
@@ -242,8 +239,8 @@ int nbh_sim_potential(nbh_sim* s, double* d_phi)
         require_fp64_walk(s, "The potential");
         const bool contains = s->cfg.theta >= kContainsTheta;
         const double eps = s->cfg.softening;
-        launch_potential_fp64(s->stream, s->w64node.p, s->w64crit.p, s->w64link.p,
-                              s->w64center.p, s->scalars.p + 3, s->rec64.p,
+        launch_potential_fp64(s->stream, s->w64.p, s->w64center.p, s->scalars.p + 3,
+                              s->rec64.p,
                               s->idx.p, static_cast<int>(s->n), eps * eps,
                               s->cfg.gravitational_constant, contains, d_phi,
                               s->errors + ERR_TRAVERSAL);

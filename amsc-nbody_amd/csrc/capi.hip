@@ -594,9 +594,9 @@ int nbh_bh_acceleration(nbh_context* ctx, const nbh_octree_view* octree,
         int* cc = ints + 2 * m;
         int* pend = ints + 3 * m;
         int* wcount = ints + 4 * m;
-        auto* wnode = static_cast<double4*>(ctx->ws.get(SLOT_TMP2, m * sizeof(double4)));
-        auto* wcrit = static_cast<double*>(ctx->ws.get(SLOT_TMP3, m * sizeof(double)));
-        auto* wlink = static_cast<int4*>(ctx->ws.get(SLOT_TMP4, m * sizeof(int4)));
+        // records of positions 0 .. m (m: the terminal record)
+        auto* w64 = static_cast<Walk64Rec*>(ctx->ws.get(SLOT_TMP2, (m + 1) * sizeof(Walk64Rec)));
+        auto* queue = static_cast<int*>(ctx->ws.get(SLOT_TMP3, kWalkQueueInts * sizeof(int)));
         const bool contains = theta >= kContainsTheta;
         auto* wcenter = contains ? static_cast<double4*>(ctx->ws.get(SLOT_TMP5, m * sizeof(double4)))
                                  : nullptr;
@@ -609,14 +609,11 @@ int nbh_bh_acceleration(nbh_context* ctx, const nbh_octree_view* octree,
         launch_pack_records(s, d_sorted_indices, d_mass, d_x, d_y, d_z, n,
                             particle_order == NBH_ORDER_MORTON, rec, err);
         launch_generic_sizes(s, o, size, local, err);
-        launch_walk_build(s, o, true, false, size, local, nullptr, theta, rec,
-                          nullptr, nullptr, contains,
-                          wnode, wcrit, wlink, wcenter, nullptr, nullptr,
-                          nullptr, wcount);
-        launch_walk_fp64(s, wnode, wcrit, wlink, wcenter, wcount, rec,
-                         d_sorted_indices, n, softening * softening,
-                         gravitational_constant, contains, d_ax, d_ay, d_az,
-                         err, nullptr);
+        launch_walk_build(s, o, size, local, theta, rec, contains, softening,
+                          gravitational_constant, w64, wcenter, wcount);
+        launch_walk_fp64(s, w64, m + 1, wcenter, wcount, rec, d_sorted_indices, nullptr,
+                         n, softening * softening, gravitational_constant, contains, 0.0,
+                         d_ax, d_ay, d_az, queue, err, nullptr);
         const int code = read_error(ctx, ERR_TRAVERSAL);
         if (code != 0)
             fail(NBH_RUNTIME_ERROR, traversal_message(code), code);

@@ -235,26 +235,52 @@ void launch_leaf_init(hipStream_t s, const nbh_octree_view& o,
                       const double* y, const double* z, int* err);
 void launch_propagate(hipStream_t s, const nbh_octree_view& o, int* err);
 
+// One position of the FP64 walk, 64 bytes, fetched by one scalar load:
+// the node's centre of mass and mass (a single particle's position and mass
+// for a one-particle leaf), the opening threshold on d^2, the link and G m.
+//   thr: the traversal takes the record at d^2 > thr.  Internal and
+//        childless nodes hold mac_threshold(crit) (exactly the reference's
+//        sqrt(d^2) > crit); one-particle leaves and zero-mass nodes hold
+//        -inf (always taken as a monopole: the leaf's exact particle term,
+//        skipping the target's own particle by index, or nothing for a zero
+//        mass); leaves of several particles and error records hold +inf.
+//   kind: KIND_INTERNAL for internal nodes and the two always-taken kinds,
+//        else the record's kind (a childless node that a lane does not take
+//        is error 5 for that lane).
+//   first, count: a leaf's particle range (count 1 for a one-particle leaf;
+//        -1, 0 for nodes).
+//   gm: G m, rounded once, as the reference's G * m * r^-3 rounds it.
+struct alignas(64) Walk64Rec {
+    double x, y, z, m;
+    double thr;
+    int skip, kind, first, count;
+    double gm;
+};
+static_assert(sizeof(Walk64Rec) == 64, "FP64 walk records are 64 bytes");
+
 // walk_layout.hip, walk_fp64.hip, walk_mixed.hip
 void launch_generic_sizes(hipStream_t s, const nbh_octree_view& o, int* size,
                           int* local, int* err);
-void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
-                       bool mixed, const int* size, const int* local,
-                       const double* crit, double theta, const double4* rec64,
-                       const int4* rec32, const double* box,
-                       bool need_center, double4* w64node, double* w64crit,
-                       int4* w64link, double4* w64center, int4* w32node,
-                       int4* w32link, int4* w32center, int* wcount);
+// The caller-octree walk build (FP64 only): w64 holds n_nodes + 1 records.
+void launch_walk_build(hipStream_t s, const nbh_octree_view& o, const int* size,
+                       const int* local, double theta, const double4* rec64,
+                       bool need_center, double eps, double G, Walk64Rec* w64,
+                       double4* w64center, int* wcount);
 int walk_blocks_padded(int n);
-void launch_walk_fp64(hipStream_t s, const double4* wnode, const double* wcrit,
-                      const int4* wlink, const double4* wcenter,
-                      const int* d_wcount, const double4* rec,
-                      const uint32_t* idx, int n, double eps2, double G,
-                      bool contains, double* ax, double* ay, double* az,
-                      int* err, unsigned long long* counters);
-// wpairs: interleaved {node, link} int4 pairs, 32 bytes per position.
 // queue: kWalkQueueInts ints of work-queue counters (cleared by the launch).
 constexpr int kWalkQueueInts = 256;
+// The FP64 traversal: w64 records (wcap of them), rec the Morton-order
+// {x, y, z, m} particles, order the sorted target indices in packet order
+// (null: Morton order).  Accelerations go to the targets' original indices.
+// side: the side of a cube holding every particle (0 when unknown), which
+// selects the walk without range tests when eps2 and side allow it.
+void launch_walk_fp64(hipStream_t s, const Walk64Rec* w64, size_t wcap,
+                      const double4* wcenter, const int* d_wcount,
+                      const double4* rec, const uint32_t* idx,
+                      const uint32_t* order, int n, double eps2, double G,
+                      bool contains, double side, double* ax, double* ay, double* az,
+                      int* queue, int* err, unsigned long long* counters);
+// wpairs: interleaved {node, link} int4 pairs, 32 bytes per position.
 // Split traversal (few packets): at most kWalkSplitMax subtrees per packet,
 // for walks of at most kWalkSplitPackets packets.  split: 1 + 4 *
 // kWalkSplitMax ints; part: kWalkSplitMax * 64 * 3 doubles per packet.

@@ -16,15 +16,15 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          bool need_center, double eps, int* size, int* local,
                          double* crit, int* pre, int* order, int* level_info,
                          double4* sum4, const int* poffset, const int* pcount,
-                         int n_groups, double4* w64node, double* w64crit,
-                         int4* w64link, double4* w64center, int4* w32node,
+                         int n_groups, double G, Walk64Rec* w64, double4* w64center,
+                         int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
                          hipEvent_t between, int* hcount = nullptr,
                          int* hoffset = nullptr, uint32_t* horder = nullptr,
                          const uint8_t* htab = nullptr);
 // ints of the level-pass bookkeeping array (level starts + per-block counts)
 size_t level_info_ints();
-void launch_walk_sentinel(hipStream_t s, int4* w32node, int4* w64link, int cap);
+void launch_walk_sentinel(hipStream_t s, int4* w32node, Walk64Rec* w64, int cap);
 template <int BITS>
 void launch_octree_views(hipStream_t s, const nbh_octree_view& o,
                          const double4* sum4, const double* box);
@@ -142,12 +142,12 @@ struct nbh_sim {
         child_count, size, local, pre, order, level_info;
     nbh::DevBuf<unsigned char> prefix;
     nbh::DevBuf<double> mass, comx, comy, comz, cenx, ceny, cenz, half, crit;
-    nbh::DevBuf<double4> w64node, w64center;
+    nbh::DevBuf<nbh::Walk64Rec> w64; // FP64 walk records
+    nbh::DevBuf<double4> w64center;
     nbh::DevBuf<double4> sum4; // packed {com, mass} per node (level passes)
     bool views_fresh = false;  // reference-layout node arrays match sum4
-    nbh::DevBuf<double> w64crit;
-    nbh::DevBuf<int4> w64link, w32node, w32center;
-    // mixed: the traversal's target order along the Hilbert curve
+    nbh::DevBuf<int4> w32node, w32center;
+    // the traversal's target order along the Hilbert curve
     nbh::DevBuf<int> hcount, hoffset;    // per node
     nbh::DevBuf<uint32_t> horder;        // per particle
     nbh::DevBuf<uint8_t> htab;           // the curve's state tables

@@ -47,13 +47,14 @@ void alloc_state(nbh_sim* s)
     s->idx_alt.alloc(n);
     s->rec64.alloc(n);
     s->pos4.alloc(n);
+    // both traversals take their packets along the Hilbert curve
+    s->horder.alloc(n);
+    const HilbertTables& tab = hilbert_tables();
+    s->htab.alloc(sizeof(HilbertTables));
+    NBH_HIP(hipMemcpyAsync(s->htab.p, &tab, sizeof(HilbertTables),
+                           hipMemcpyHostToDevice, s->stream));
     if (mixed(s)) {
         s->rec32.alloc(n);
-        s->horder.alloc(n);
-        const HilbertTables& tab = hilbert_tables();
-        s->htab.alloc(sizeof(HilbertTables));
-        NBH_HIP(hipMemcpyAsync(s->htab.p, &tab, sizeof(HilbertTables),
-                               hipMemcpyHostToDevice, s->stream));
         if (walk_mixed_splits(static_cast<int>(n))) {
             s->wsplit.alloc(kWalkSplitInts);
             s->wpart.alloc((n + 63) / 64 * kWalkSplitMax * 64 * 3);
@@ -117,23 +118,21 @@ void ensure_octree(nbh_sim* s, size_t m)
     for (auto* b : {&s->mass, &s->comx, &s->comy, &s->comz, &s->cenx, &s->ceny,
                     &s->cenz, &s->half, &s->crit})
         b->alloc(cap);
+    s->hcount.alloc(cap);
+    s->hoffset.alloc(cap);
     if (mixed(s)) {
-        s->hcount.alloc(cap);
-        s->hoffset.alloc(cap);
         s->w32node.alloc(2 * cap); // interleaved {node, link} pairs
         if (s->cfg.theta >= kContainsTheta)
             s->w32center.alloc(cap);
     } else {
-        s->w64node.alloc(cap);
-        s->w64crit.alloc(cap);
-        s->w64link.alloc(cap);
+        s->w64.alloc(cap);
         if (s->cfg.theta >= kContainsTheta)
             s->w64center.alloc(cap);
     }
     if (cap > 0x7FFFFFFFull)
         fail(NBH_OVERFLOW_ERROR, "octree node count exceeds the walk's position range");
     launch_walk_sentinel(s->stream, mixed(s) ? s->w32node.p : nullptr,
-                         mixed(s) ? nullptr : s->w64link.p, static_cast<int>(cap));
+                         mixed(s) ? nullptr : s->w64.p, static_cast<int>(cap));
     s->mcap = cap;
 }
 
@@ -223,8 +222,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
         st, o, Layout<BITS>::bits, mixed(s), s->rec64.p, s->rec32.p,
         s->cfg.theta, s->box.p, contains,
         s->cfg.softening, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->sum4.p, s->poffset.p, s->pcount.p, s->ngroups,
-        s->w64node.p, s->w64crit.p,
-        s->w64link.p, s->w64center.p, s->w32node.p, nullptr,
+        s->cfg.gravitational_constant, s->w64.p, s->w64center.p, s->w32node.p, nullptr,
         s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr,
         s->hcount.p, s->hoffset.p, s->horder.p, s->htab.p);
 }
@@ -250,11 +248,10 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters)
                           a[0].p, a[1].p, a[2].p, s->walkq.p, s->wsplit.p, s->wpart.p,
                           s->errors + ERR_TRAVERSAL, counters);
     } else {
-        launch_walk_fp64(st, s->w64node.p, s->w64crit.p, s->w64link.p,
-                         s->w64center.p, s->scalars.p + 3, s->rec64.p, s->idx.p,
-                         ni, eps * eps, s->cfg.gravitational_constant, contains,
-                         a[0].p, a[1].p, a[2].p, s->errors + ERR_TRAVERSAL,
-                         counters);
+        launch_walk_fp64(st, s->w64.p, s->mcap, s->w64center.p, s->scalars.p + 3,
+                         s->rec64.p, s->idx.p, s->horder.p, ni, eps * eps,
+                         s->cfg.gravitational_constant, contains, s->rb->box[3], a[0].p,
+                         a[1].p, a[2].p, s->walkq.p, s->errors + ERR_TRAVERSAL, counters);
     }
 }
 
@@ -569,7 +566,7 @@ int nbh_sim_structure_view(nbh_sim* s, nbh_sim_structure* out)
         out->radix_level = s->plevel.p;
         out->edge_node_count = s->pcount.p;
         out->edge_node_offset = s->poffset.p;
-        out->target_order = mixed(s) ? s->horder.p : nullptr;
+        out->target_order = s->horder.p;
         out->box = s->box.p;
     });
 }

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "device_utils.hpp"
+#include "mac_threshold.hpp"
 #include "nbh_internal.hpp"
 
 namespace nbh {
@@ -22,7 +23,90 @@ enum : int {
 constexpr int kMixedLeafCountLimit = 1 << 23;
 constexpr int kErrMixedLeafTooLarge = 9;
 
-constexpr int kWalkThreads = 256; // 4 independent wavefronts per block
+
+// "Error 4 (inconsistent topology), skip to `skip`": the FP64 walk's
+// terminal record at the walk length and the initial content of every
+// record buffer position.
+__host__ __device__ inline Walk64Rec error_record64(int skip)
+{
+    Walk64Rec r;
+    r.x = r.y = r.z = r.m = 0.0;
+    r.thr = INFINITY;
+    r.skip = skip;
+    r.kind = KIND_ERROR + 4;
+    r.first = -1;
+    r.count = 0;
+    r.gm = 0.0;
+    return r;
+}
+
+constexpr int kWalkThreads = 256;
+
+// ints between the work-queue counters of two ranges (one per 128 bytes)
+constexpr int kQueueStride = kWalkQueueInts / 8;
+
+// The work queue of the persistent traversals: packets (units) are dealt
+// from eight ranges, one eighth of the target order each, through one
+// counter per range.  A wavefront takes packets from the range of its own
+// XCD, so each XCD's L2 serves one region of space, and from the other
+// ranges once its own is empty.
+struct Queue {
+    int* counters; // counter of range r at counters[r * kQueueStride]
+    int packets;
+    int per;       // packets per range
+    int home;      // this wavefront's XCD
+    int tried;     // ranges found empty so far, in order home, home + 1, ...
+
+    __device__ __forceinline__ int take()
+    {
+        while (tried < 8) {
+            const int r = (home + tried) & 7;
+            const int lo = r * per;
+            const int hi = min(lo + per, packets);
+            int k = 0;
+            if ((threadIdx.x & 63) == 0)
+                k = atomicAdd(&counters[r * kQueueStride], 1);
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (lo + k < hi)
+                return lo + k;
+            ++tried;
+        }
+        return -1;
+    }
+};
+
+__device__ __forceinline__ int xcd_id()
+{
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return static_cast<int>(v & 7u);
+}
+
+namespace {
+
+__global__ void queue_clear_kernel(int* __restrict__ queue)
+{
+    if (threadIdx.x < 8)
+        queue[threadIdx.x * kQueueStride] = 0;
+}
+
+} // namespace
+
+// Blocks of 4 persistent wavefronts: one wavefront per resident slot, 32
+// per CU (8 per SIMD), fewer when there are fewer pairs of units.
+inline int walk_persistent_blocks(long long units)
+{
+    static int cus[64] = {};
+    int dev = 0;
+    NBH_HIP(hipGetDevice(&dev));
+    int& c = cus[dev & 63];
+    if (c == 0)
+        NBH_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
+    const long long pairs = (units + 1) / 2;
+    const long long cap = 32LL * c;
+    const int waves = static_cast<int>(pairs < cap ? pairs : cap);
+    return (waves + 3) / 4;
+} // 4 independent wavefronts per block
 
 // Static grids (the FP64 walks): packets are Morton-consecutive; blocks are
 // dealt to the 8 XCDs round

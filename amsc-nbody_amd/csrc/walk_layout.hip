@@ -154,13 +154,12 @@ __device__ __forceinline__ int curve_state(typename Layout<BITS>::Key p, int L,
 }
 
 struct WalkOut {
-    double4* w64node;
-    double* w64crit;
-    int4* w64link;
+    Walk64Rec* w64;
     double4* w64center;
     int4* w32node;
     int4* w32link;
     int4* w32center;
+    double G; // FP64 records hold G m
 };
 
 // The record of node v at pre-order position `pre`.  GENERIC: a
@@ -176,9 +175,6 @@ __device__ __forceinline__ void write_record(
     const double* __restrict__ box, bool need_center, double eps,
     const WalkOut& w, const double4* __restrict__ sum4, double4 geom)
 {
-    double4* w64node = w.w64node;
-    double* w64crit = w.w64crit;
-    int4* w64link = w.w64link;
     double4* w64center = w.w64center;
     int4* w32node = w.w32node;
     int4* w32link = w.w32link;
@@ -240,16 +236,46 @@ __device__ __forceinline__ void write_record(
         }
     }
     if (!MIXED) {
-        if (kind == KIND_LEAF && count == 1)
-            w64node[pre] = rec64[first];
-        else
-            w64node[pre] = make_double4(cx, cy, cz, m);
-        w64crit[pre] = crit;
-        w64link[pre] = make_int4(skip, kind, first, count);
-        if (need_center)
-            w64center[pre] = GENERIC ? make_double4(o.center_x[v], o.center_y[v],
-                                                    o.center_z[v], h)
-                                     : geom;
+        // One-particle leaves and zero-mass nodes are always-taken
+        // monopoles (thr = -inf): a leaf's exact particle term (the walk
+        // skips the target's own particle by index) or nothing for a zero
+        // mass; the cell test of large theta never applies to them (half -1).
+        Walk64Rec r;
+        const bool fast = (kind == KIND_LEAF && count == 1) || kind == KIND_SKIP;
+        if (kind == KIND_LEAF && count == 1) {
+            const double4 p = rec64[first];
+            r.x = p.x;
+            r.y = p.y;
+            r.z = p.z;
+            r.m = p.w;
+        } else {
+            r.x = cx;
+            r.y = cy;
+            r.z = cz;
+            r.m = m;
+        }
+        r.skip = skip;
+        r.first = (kind == KIND_LEAF) ? first : -1;
+        r.count = (kind == KIND_LEAF) ? count : 0;
+        r.gm = w.G * r.m;
+        if (fast) {
+            r.kind = KIND_INTERNAL;
+            r.thr = -INFINITY;
+        } else if (kind == KIND_INTERNAL || kind == KIND_CHILDLESS) {
+            r.kind = kind;
+            r.thr = mac_threshold(crit);
+        } else {
+            r.kind = kind;
+            r.thr = INFINITY;
+        }
+        w.w64[pre] = r;
+        if (need_center) {
+            double4 c = GENERIC ? make_double4(o.center_x[v], o.center_y[v], o.center_z[v], h)
+                                : geom;
+            if (fast)
+                c.w = -1.0;
+            w64center[pre] = c;
+        }
     } else {
         const double side = box[3];
         const double scale = 2147483648.0 / side;
@@ -312,7 +338,7 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
     const int* __restrict__ local, const double* __restrict__ crit_in,
     double theta, const double4* __restrict__ rec64,
     const int4* __restrict__ rec32, const double* __restrict__ box,
-    bool need_center, WalkOut w, int* __restrict__ wcount)
+    bool need_center, double eps, WalkOut w, int* __restrict__ wcount)
 {
     const int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (v >= o.n_nodes)
@@ -323,7 +349,7 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
     if (v == 0)
         *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
     write_record<GENERIC, MIXED>(o, v, pre, size, crit_in, theta, rec64, rec32,
-                                 box, need_center, 0.0, w, nullptr,
+                                 box, need_center, eps, w, nullptr,
                                  make_double4(0.0, 0.0, 0.0, 0.0));
 }
 
@@ -488,7 +514,7 @@ __global__ __launch_bounds__(256) void level_down_kernel(
                 w.w32link[2 * static_cast<size_t>(wc)] =
                     make_int4(0x7F800000, wc, KIND_ERROR + 4, 0);
             else
-                w.w64link[wc] = make_int4(wc, KIND_ERROR + 4, 0, 0);
+                w.w64[wc] = error_record64(wc);
         } else {
             const int parent = o.parent[v];
             pre = pre_of[parent] + local[v];
@@ -752,6 +778,15 @@ __global__ __launch_bounds__(256) void link_sentinel_kernel(int4* __restrict__ l
         link[static_cast<size_t>(i) * stride] = v;
 }
 
+// The same for FP64 records: n records "error 4, skip to `skip`".
+__global__ __launch_bounds__(256) void record_sentinel_kernel(Walk64Rec* __restrict__ w,
+                                                              int n, int skip)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+        w[i] = error_record64(skip);
+}
+
 } // namespace
 
 void launch_generic_sizes(hipStream_t s, const nbh_octree_view& o, int* size,
@@ -764,29 +799,21 @@ void launch_generic_sizes(hipStream_t s, const nbh_octree_view& o, int* size,
     NBH_HIP(hipGetLastError());
 }
 
-void launch_walk_build(hipStream_t s, const nbh_octree_view& o, bool generic,
-                       bool mixed, const int* size, const int* local,
-                       const double* crit, double theta, const double4* rec64,
-                       const int4* rec32, const double* box,
-                       bool need_center, double4* w64node, double* w64crit,
-                       int4* w64link, double4* w64center, int4* w32node,
-                       int4* w32link, int4* w32center, int* wcount)
+// Positions [0, M] start as "error 4, skip to M", position M stays the
+// terminal record, and every node on the root's tree writes its own.
+void launch_walk_build(hipStream_t s, const nbh_octree_view& o, const int* size,
+                       const int* local, double theta, const double4* rec64,
+                       bool need_center, double eps, double G, Walk64Rec* w64,
+                       double4* w64center, int* wcount)
 {
-    if (!generic || mixed)
-        fail(NBH_LOGIC_ERROR,
-             "the per-node walk build serves caller octrees in FP64 only");
-    (void)w32node;
-    (void)w32link;
-    (void)w32center;
-    const unsigned blocks = blocks_for(o.n_nodes, 256);
     const int M = o.n_nodes;
-    link_sentinel_kernel<<<blocks, 256, 0, s>>>(
-        w64link, make_int4(M, KIND_ERROR + 4, 0, 0), M, 1);
+    record_sentinel_kernel<<<blocks_for(static_cast<size_t>(M) + 1, 256), 256, 0, s>>>(
+        w64, M + 1, M);
     NBH_HIP(hipGetLastError());
-    const WalkOut w{w64node, w64crit, w64link, w64center, nullptr, nullptr,
-                    nullptr};
-    walk_build_kernel<true, false><<<blocks, 256, 0, s>>>(
-        o, size, local, crit, theta, rec64, rec32, box, need_center, w, wcount);
+    const WalkOut w{w64, w64center, nullptr, nullptr, nullptr, G};
+    walk_build_kernel<true, false><<<blocks_for(M, 256), 256, 0, s>>>(
+        o, size, local, nullptr, theta, rec64, nullptr, nullptr, need_center, eps, w,
+        wcount);
     NBH_HIP(hipGetLastError());
 }
 
@@ -800,8 +827,7 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
                          bool need_center, double eps, int* size, int* local, double* crit,
                          int* pre, int* order, int* level_info, double4* sum4,
                          const int* poffset, const int* pcount, int n_groups,
-                         double4* w64node, double* w64crit,
-                         int4* w64link, double4* w64center, int4* w32node,
+                         double G, Walk64Rec* w64, double4* w64center, int4* w32node,
                          int4* w32link, int4* w32center, int* wcount,
                          hipEvent_t between, int* hcount, int* hoffset,
                          uint32_t* horder, const uint8_t* htab)
@@ -849,8 +875,8 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     // The error sentinel of the other positions is written once, when the
     // record buffers are allocated (launch_walk_sentinel); the buffers hold
     // at least n_nodes + 1024 records.
-    const WalkOut w{w64node, w64crit, w64link, w64center, w32node, w32pair_link,
-                    w32center};
+    const WalkOut w{w64, w64center, w32node, w32pair_link,
+                    w32center, G};
     for (int L = 0; L <= max_level; ++L) {
         if (mixed)
             level_down_kernel<BITS, true><<<blocks, 256, 0, s>>>(
@@ -881,14 +907,14 @@ size_t level_info_ints() { return kLevelInfoInts; }
 // freshly allocated walk records.  Records are later overwritten only by
 // the layout passes, whose skip links always point forward, so a walk over
 // these buffers always ends.
-void launch_walk_sentinel(hipStream_t s, int4* w32node, int4* w64link, int cap)
+void launch_walk_sentinel(hipStream_t s, int4* w32node, Walk64Rec* w64, int cap)
 {
     if (w32node)
         link_sentinel_kernel<<<blocks_for(cap, 256), 256, 0, s>>>(
             w32node + 1, make_int4(0x7F800000, cap, KIND_ERROR + 4, 0), cap, 2);
-    if (w64link)
-        link_sentinel_kernel<<<blocks_for(cap, 256), 256, 0, s>>>(
-            w64link, make_int4(cap, KIND_ERROR + 4, 0, 0), cap, 1);
+    // FP64: skip to the last position, at or beyond which every walk ends
+    if (w64)
+        record_sentinel_kernel<<<blocks_for(cap, 256), 256, 0, s>>>(w64, cap, cap - 1);
     NBH_HIP(hipGetLastError());
 }
 
@@ -912,8 +938,7 @@ template void launch_octree_views<21>(hipStream_t, const nbh_octree_view&,
                                          const double4*, const int4*, double,           \
                                          const double*, bool, double, int*, int*,       \
                                          double*, int*, int*, int*, double4*,           \
-                                         const int*, const int*, int, double4*,          \
-                                         double*, int4*,                                 \
+                                         const int*, const int*, int, double, Walk64Rec*, \
                                          double4*, int4*, int4*, int4*, int*, hipEvent_t, \
                                          int*, int*, uint32_t*, const uint8_t*);
 NBH_LEVEL_INST(10)

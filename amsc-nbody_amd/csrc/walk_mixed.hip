@@ -45,8 +45,6 @@ namespace {
 constexpr int kFold = 64;
 // Kind-byte flag of the records that start a split walk's subtrees.
 constexpr int kBoundary = 0x80;
-// ints between the work-queue counters of two ranges (one per 128 bytes)
-constexpr int kQueueStride = kWalkQueueInts / 8;
 
 // One position of the mixed walk: the node (fixed-point COM, m / m_max) and
 // its link (crit^2 + eps^2, skip, kind | count << 8, first particle).
@@ -519,42 +517,6 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
     az_out[o] = out_scale * w.a.kz.value();
 }
 
-// The work queue: packets are dealt from eight ranges, one eighth of the
-// target order each, through one counter per range.  A wavefront takes
-// packets from the range of its own XCD, so each XCD's L2 serves one region
-// of space, and from the other ranges once its own is empty.
-struct Queue {
-    int* counters; // counter of range r at counters[r * kQueueStride]
-    int packets;
-    int per;       // packets per range
-    int home;      // this wavefront's XCD
-    int tried;     // ranges found empty so far, in order home, home + 1, ...
-
-    __device__ __forceinline__ int take()
-    {
-        while (tried < 8) {
-            const int r = (home + tried) & 7;
-            const int lo = r * per;
-            const int hi = min(lo + per, packets);
-            int k = 0;
-            if ((threadIdx.x & 63) == 0)
-                k = atomicAdd(&counters[r * kQueueStride], 1);
-            k = __builtin_amdgcn_readfirstlane(k);
-            if (lo + k < hi)
-                return lo + k;
-            ++tried;
-        }
-        return -1;
-    }
-};
-
-__device__ __forceinline__ int xcd_id()
-{
-    unsigned v;
-    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-    return static_cast<int>(v & 7u);
-}
-
 // Persistent wavefronts, each with two unit slots walked in lock-step: one
 // iteration loads both slots' records, then does both visits, so the two
 // record loads (mostly served by L2: the walk's working set exceeds the
@@ -808,28 +770,7 @@ __global__ __launch_bounds__(256) void walk_split_sum_kernel(
     az_out[o] = out_scale * sz;
 }
 
-__global__ void queue_clear_kernel(int* __restrict__ queue)
-{
-    if (threadIdx.x < 8)
-        queue[threadIdx.x * kQueueStride] = 0;
-}
-
 } // namespace
-
-// One wavefront per resident slot: 32 per CU (8 per SIMD), fewer when
-// there are fewer pairs of units.
-static int walk_persistent_blocks(long long units)
-{
-    static int cus[64] = {};
-    int dev = 0;
-    NBH_HIP(hipGetDevice(&dev));
-    int& c = cus[dev & 63];
-    if (c == 0)
-        NBH_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
-    const long long pairs = (units + 1) / 2;
-    const int waves = static_cast<int>(std::min<long long>(pairs, 32LL * c));
-    return (waves + 3) / 4;
-}
 
 bool walk_mixed_splits(int n)
 {

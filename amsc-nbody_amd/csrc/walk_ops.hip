@@ -119,8 +119,7 @@ struct Lane {
 
 template <bool CONTAINS, int OP>
 __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
-    const double4* __restrict__ wnode, const double* __restrict__ wcrit,
-    const int4* __restrict__ wlink, const double4* __restrict__ wcenter,
+    const Walk64Rec* __restrict__ w64, const double4* __restrict__ wcenter,
     const int* __restrict__ d_wcount, const double4* __restrict__ rec,
     const uint32_t* __restrict__ idx, int n, int nblocks_padded, double eps2,
     double G, OpArgs a, int* err)
@@ -154,18 +153,27 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
     int next = valid ? 0 : 0x7FFFFFFF;
     int i = 0;
     while (i < wcount) {
-        const int4 lk = wlink[i];
-        const int skip = lk.x;
-        const int kind = lk.y;
+        const Walk64Rec r = w64[i];
+        const int skip = r.skip;
+        const int kind = r.kind;
         const bool active = next <= i && L.code == 0;
         bool open = false;
-        if (kind == KIND_INTERNAL || kind == KIND_CHILDLESS) {
-            const double4 nd = wnode[i];
-            const double dx = nd.x - L.tx;
-            const double dy = nd.y - L.ty;
-            const double dz = nd.z - L.tz;
+        // a record with thr = -inf is a one-particle leaf (count 1) or a
+        // zero-mass node (count 0), taken without an opening test
+        const bool fast = kind == KIND_INTERNAL && r.thr == -INFINITY;
+        if (fast) {
+            if (r.count == 1)
+                L.source(active && r.first != t, r.x, r.y, r.z, r.m, G, eps2, a.pbx,
+                         a.pby, a.pbz, a.pm, r.first);
+            if (active)
+                next = skip;
+        } else if (kind == KIND_INTERNAL || kind == KIND_CHILDLESS) {
+            const double dx = r.x - L.tx;
+            const double dy = r.y - L.ty;
+            const double dz = r.z - L.tz;
             const double d2 = fma(dz, dz, fma(dx, dx, dy * dy));
-            bool accept = active && sqrt(d2) > wcrit[i];
+            // the opening test sqrt(d2) > crit as d2 > its exact threshold
+            bool accept = active && d2 > r.thr;
             if (CONTAINS && accept) {
                 const double4 c = wcenter[i];
                 const bool inside = fabs(L.tx - c.x) <= c.w &&
@@ -183,25 +191,14 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
                     next = i + 1;
                 }
             }
-            L.source(accept, nd.x, nd.y, nd.z, nd.w, G, eps2, a.nbx, a.nby,
+            L.source(accept, r.x, r.y, r.z, r.m, G, eps2, a.nbx, a.nby,
                      a.nbz, a.nm, i);
         } else if (kind == KIND_LEAF) {
-            const int first = lk.z;
-            const int count = lk.w;
-            if (count == 1) {
-                const double4 s = wnode[i];
-                L.source(active && first != t, s.x, s.y, s.z, s.w, G, eps2,
-                         a.pbx, a.pby, a.pbz, a.pm, first);
-            } else {
-                for (int k = first; k < first + count; ++k) {
-                    const double4 s = rec[k];
-                    L.source(active && k != t, s.x, s.y, s.z, s.w, G, eps2,
-                             a.pbx, a.pby, a.pbz, a.pm, k);
-                }
+            for (int k = r.first; k < r.first + r.count; ++k) {
+                const double4 s = rec[k];
+                L.source(active && k != t, s.x, s.y, s.z, s.w, G, eps2,
+                         a.pbx, a.pby, a.pbz, a.pm, k);
             }
-            if (active)
-                next = skip;
-        } else if (kind == KIND_SKIP) {
             if (active)
                 next = skip;
         } else if (active) {
@@ -337,20 +334,17 @@ __global__ __launch_bounds__(256) void energy_partials_kernel(
 }
 
 template <int OP>
-void launch_op(hipStream_t s, const double4* wnode, const double* wcrit,
-               const int4* wlink, const double4* wcenter, const int* d_wcount,
-               const double4* rec, const uint32_t* idx, int n, double eps2,
-               double G, bool contains, const OpArgs& a, int* err)
+void launch_op(hipStream_t s, const Walk64Rec* w64, const double4* wcenter,
+               const int* d_wcount, const double4* rec, const uint32_t* idx, int n,
+               double eps2, double G, bool contains, const OpArgs& a, int* err)
 {
     const int nb = walk_blocks_padded(n);
     if (contains)
         walk_op_kernel<true, OP><<<nb, kWalkThreads, 0, s>>>(
-            wnode, wcrit, wlink, wcenter, d_wcount, rec, idx, n, nb, eps2, G, a,
-            err);
+            w64, wcenter, d_wcount, rec, idx, n, nb, eps2, G, a, err);
     else
         walk_op_kernel<false, OP><<<nb, kWalkThreads, 0, s>>>(
-            wnode, wcrit, wlink, wcenter, d_wcount, rec, idx, n, nb, eps2, G, a,
-            err);
+            w64, wcenter, d_wcount, rec, idx, n, nb, eps2, G, a, err);
     NBH_HIP(hipGetLastError());
 }
 
@@ -358,8 +352,7 @@ void launch_op(hipStream_t s, const double4* wnode, const double* wcrit,
 
 int walk_blocks_padded(int n);
 
-void launch_potential_fp64(hipStream_t s, const double4* wnode,
-                           const double* wcrit, const int4* wlink,
+void launch_potential_fp64(hipStream_t s, const Walk64Rec* w64,
                            const double4* wcenter, const int* d_wcount,
                            const double4* rec, const uint32_t* idx, int n,
                            double eps2, double G, bool contains, double* phi,
@@ -367,8 +360,8 @@ void launch_potential_fp64(hipStream_t s, const double4* wnode,
 {
     OpArgs a{};
     a.phi = phi;
-    launch_op<OP_POTENTIAL>(s, wnode, wcrit, wlink, wcenter, d_wcount, rec, idx,
-                            n, eps2, G, contains, a, err);
+    launch_op<OP_POTENTIAL>(s, w64, wcenter, d_wcount, rec, idx, n, eps2, G, contains,
+                            a, err);
 }
 
 void launch_energy_partials(hipStream_t s, const double* m, const double* vx,
@@ -382,8 +375,7 @@ void launch_energy_partials(hipStream_t s, const double* m, const double* vx,
 
 void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
                            int max_level, const int* pre_of,
-                           const double4* wnode, const double* wcrit,
-                           const int4* wlink, const double4* wcenter,
+                           const Walk64Rec* w64, const double4* wcenter,
                            const int* d_wcount, int wcount_host,
                            const double4* rec, const uint32_t* idx, int n,
                            double eps2, double G, bool contains,
@@ -418,8 +410,8 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
     a.nbz = nbar + 2 * W;
     a.pm = pm;
     a.nm = nm;
-    launch_op<OP_VJP>(s, wnode, wcrit, wlink, wcenter, d_wcount, rec, idx, n,
-                      eps2, G, contains, a, err);
+    launch_op<OP_VJP>(s, w64, wcenter, d_wcount, rec, idx, n, eps2, G, contains, a,
+                      err);
     unsigned blocks = blocks_for(M, 256);
     if (blocks > 2048)
         blocks = 2048;

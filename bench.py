@@ -450,6 +450,8 @@ def main():
             "fp64_rel_drift": energy64, "fp64_steps": args.fp64_steps},
         "accuracy_by_theta": by_theta,
         "phase_ms": {k: round(v, 4) for k, v in phases.items()},
+        "fp64_phase_ms": None if fp64_rate is None else
+        {k: round(v, 4) for k, v in phases64.items()},
         "frame_capture": frames,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -311,10 +311,9 @@ typedef struct nbh_sim_structure {
     const int* edge_node_count;
     const int* edge_node_offset;
     const double* box; /* device [cx, cy, cz, side] */
-    /* Mixed precision: the sorted indices in the traversal's packet order,
-     * along a 3-D Hilbert curve through the 21-bit grid positions (no
-     * counterpart in the reference, which walks one target per thread);
-     * null in FP64 mode. */
+    /* The sorted indices in the traversal's packet order, along a 3-D
+     * Hilbert curve through the grid positions of the keys (no counterpart
+     * in the reference, which walks one target per thread). */
     const uint32_t* target_order;
 } nbh_sim_structure;
 int nbh_sim_structure_view(nbh_sim* sim, nbh_sim_structure* out);

@@ -21,3 +21,17 @@ def test_cpp_drop_in_layer():
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
     assert r.stdout.count("PASS") == 9
+
+
+ARITH_BIN = os.path.join(ROOT, "amsc-nbody_amd", "cpp", "tests", "bin", "fp64_arith_test")
+
+
+def test_fp64_walk_arithmetic_equals_ieee_operations():
+    """The FP64 traversal's reduced square root and reciprocal equal the
+    device's IEEE sqrt() and 1.0 / x bit for bit in their range
+    (cpp/tests/fp64_arith_test.hip: random operands and near-ties)."""
+    assert os.path.exists(ARITH_BIN), "build the C++ tests first (make -C amsc-nbody_amd cpp-tests)"
+    r = subprocess.run([ARITH_BIN], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASS" in r.stdout

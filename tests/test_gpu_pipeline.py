@@ -106,15 +106,17 @@ def test_mixed_walk_ragged_sizes(nb, orc, n):
     assert rms < MIXED_RMS and mx < MIXED_MAX
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
 @pytest.mark.parametrize("kind,n", [("plummer", 20_000), ("clumps", 30_000),
                                     ("uniform", 4096), ("plummer", 1), ("plummer", 65)])
-def test_mixed_target_order_follows_hilbert_curve(nb, kind, n):
-    """The traversal's packets take the sorted particles along the Hilbert
-    curve through their 21-bit grid positions (ties in sorted order): a
-    permutation equal to a stable sort by the curve index."""
+def test_target_order_follows_hilbert_curve(nb, kind, n, precision):
+    """Both traversals take their packets' targets along the Hilbert curve
+    through the 21-bit grid positions of the sorted particles (ties in
+    sorted order): a permutation equal to a stable sort by the curve index."""
     p = make(nb, kind, n, seed=11)
+    prec = nb.PRECISION_MIXED if precision == "mixed" else nb.PRECISION_FP64
     sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
-                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+                        precision=prec, bits_per_axis=21)
     sim.initialize()
     s = sim.structure_host()
     order = s["target_order"]
@@ -123,15 +125,6 @@ def test_mixed_target_order_follows_hilbert_curve(nb, kind, n):
     qx, qy, qz = H.morton_decode(s["keys"], 21)
     hk = H.hilbert_keys(qx, qy, qz, 21)
     assert np.array_equal(order, np.argsort(hk, kind="stable").astype(np.uint32))
-    sim.close()
-
-
-def test_fp64_has_no_target_order(nb):
-    p = make(nb, "plummer", 1000)
-    sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
-                        precision=nb.PRECISION_FP64, bits_per_axis=21)
-    sim.initialize()
-    assert sim.structure_host()["target_order"] is None
     sim.close()
 
 
