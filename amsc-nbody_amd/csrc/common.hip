@@ -6,17 +6,6 @@
 
 namespace nbh {
 
-namespace {
-thread_local std::string t_message;
-thread_local int t_device_code = 0;
-} // namespace
-
-void set_last_error(int code, const std::string& message)
-{
-    t_device_code = code;
-    t_message = message;
-}
-
 [[noreturn]] void fail(int status, const std::string& message, int code)
 {
     throw Error{status, code, message};
@@ -121,9 +110,6 @@ void launch_fill_u64(hipStream_t s, unsigned long long* p,
 
 extern "C" {
 
-const char* nbh_last_error_message(void) { return nbh::t_message.c_str(); }
-int nbh_last_device_code(void) { return nbh::t_device_code; }
-const char* nbh_version(void) { return "nbody_amd 0.1 gfx950"; }
 
 int nbh_context_create(nbh_context** out, void* hip_stream)
 {

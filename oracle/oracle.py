@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# ORACLE_LIB selects another build of the same source (the sanitizer builds
+# of oracle/Makefile); the default is the optimised build beside this file.
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 class OracleError(RuntimeError):
@@ -26,6 +28,8 @@ class OracleError(RuntimeError):
 
 
 def build(force=False):
+    if "ORACLE_LIB" in os.environ:
+        return
     if force or not os.path.exists(LIB_PATH):
         subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "liboracle.so"],
                        check=True, stdout=subprocess.DEVNULL)
