@@ -1,17 +1,28 @@
 """Benchmark: Barnes-Hut leapfrog steps on one MI355X.
 
-Workload (default): Plummer sphere, N = 4,194,304, theta 0.5, softening 1e-3,
-dt 1e-4, G = 1, 63-bit Morton keys.  One step = kick-drift + bounding box,
-Morton keys, radix sort, leaf groups, Karras tree, radix->octree plan,
-octree, mass/COM, walk layout, traversal, closing kick -- the reference's
-Simulation::step (src/simulation.cpp:100-119) -- with inputs resident in HBM.
+Headline workload (the measured metric): Plummer sphere, N = 4,194,304,
+theta 0.5, softening 1e-3, dt 1e-4, G = 1, 63-bit Morton keys.  One step =
+kick-drift + bounding box, Morton keys, radix sort, leaf groups, Karras tree,
+radix->octree plan, octree, mass/COM, walk layout, traversal, closing kick --
+the reference's Simulation::step (src/simulation.cpp:100-119) -- with inputs
+resident in HBM.  `value` is the mixed-FP32 pipeline's Mparticle-steps/s over
+exactly --steps timed steps after --warmup steps, with the per-phase timing
+events off; the phase breakdown comes from separate steps that record them.
 
-Prints one JSON line: Mparticle-steps/s of the mixed-FP32 pipeline, the FP64
-pipeline's rate and the speed-up, the accuracy of the mixed forces against
-the FP64 pipeline and an FP64 direct sum (at theta 0.3 / 0.5 / 0.8 for the
-Plummer workload), momentum and energy drift over the timed steps, the
-traversal's achieved VALU FLOP/s against the MI355X FP32 peak, and the CPU
-oracle timed on a bounded sample of the same workload.
+The same JSON line carries:
+  - the plain FP64 pipeline on the same workload (rate, phases, roofline)
+    and the mixed/FP64 ratio;
+  - accuracy of the mixed forces against the FP64 pipeline (bit-exact with
+    the oracle) and against an FP64 direct sum on 8192 targets, and energy /
+    momentum drift over 50 steps, at theta 0.3, 0.5 and 0.8;
+  - `configs`: the other workloads of the benchmark set -- 1M uniform FP64
+    (20 steps, tree checked against the oracle, 4096 direct-sum targets),
+    16M clumps mixed (20 steps, phase breakdown with per-phase HBM GB/s),
+    the reference's own benchmark blob at 100K and 250K particles, and the
+    differentiable disk + bulge workload;
+  - the traversal roofline (VALU FLOP/s against the MI355X peaks), with the
+    traffic figure taken from the committed rocprofv3 counter profile;
+  - the CPU oracle timed on the host on bounded samples (median of 3).
 
 Energy: kinetic + potential, the potential from an FP64 walk over a tree
 built at the state (Simulation.energy of an FP64 simulation of that state).
@@ -21,6 +32,9 @@ import argparse
 import json
 import math
 import os
+import platform
+import statistics
+import subprocess
 import sys
 import time
 
@@ -34,6 +48,15 @@ FP32_VALU_PEAK_TFLOPS = 157.3
 FP64_VALU_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_INTERACTION = 20
+TRAFFIC_PROFILE = os.path.join("profiles", "walk_mixed_plummer4m_traffic.md")
+
+
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def rel_err(a, b):
@@ -43,15 +66,30 @@ def rel_err(a, b):
     return float(np.sqrt((r ** 2).mean())), float(r.max())
 
 
+def g4(x):
+    return None if x is None else float(f"{x:.4g}")
+
+
 def momentum(s):
     return np.array([math.fsum(s["m"] * s[k]) for k in ("vx", "vy", "vz")])
 
 
-def time_steps(nb, torch, sim, steps):
-    """Runs `steps` steps; returns seconds and mean phase times (ms)."""
-    phases = None
+def time_steps(torch, sim, steps):
+    """Seconds for `steps` steps with the per-phase timing events off."""
+    sim.set_timing(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def phase_steps(sim, steps=2):
+    """Mean per-phase times (ms) of `steps` steps that record HIP events
+    between the phases (not part of the timed steps)."""
+    sim.set_timing(True)
+    phases = None
     for _ in range(steps):
         sim.step()
         pt = sim.phase_times()
@@ -59,17 +97,44 @@ def time_steps(nb, torch, sim, steps):
             phases = {k: 0.0 for k in pt}
         for k, v in pt.items():
             phases[k] += v / steps
-    torch.cuda.synchronize()
-    return time.perf_counter() - t0, phases
+    sim.set_timing(False)
+    return {k: round(v, 4) for k, v in phases.items()}
+
+
+def algorithmic_bytes(n, g, m):
+    """Algorithmic HBM bytes per phase of one step: N particles, G leaf
+    groups, R = 2G - 1 radix nodes, M octree nodes; 63-bit keys (8 B),
+    indices 4 B, FP64 state."""
+    r = 2 * g - 1
+    return {
+        "kick_drift_bbox": 120 * n, "encode": 36 * n, "sort": 200 * n, "gather": 84 * n,
+        "groups": 8 * n + 16 * g, "karras": 8 * g + 20 * (g - 1) + 4 * r, "plan": 24 * r,
+        "octree": 24 * r + 56 * m, "com": 36 * n + 100 * m, "walk_layout": 92 * m,
+        "kick": 72 * n,
+    }
+
+
+def phase_bandwidth(phases, n, g, m):
+    out = {}
+    for k, b in algorithmic_bytes(n, g, m).items():
+        ms = phases.get(k, 0.0)
+        if ms > 0:
+            out[k] = {"ms": ms, "GB_per_s": round(b / (ms * 1e-3) / 1e9, 1),
+                      "frac_of_hbm_peak": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def sizes(sim):
+    s = sim.structure()
+    return s.n_groups, s.n_octree_nodes
 
 
 def traversal_traffic():
-    """HBM bytes per traversal dispatch from the committed counter summary
+    """HBM bytes per traversal dispatch from the committed counter profile
     (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, calibrated against a copy of
     known size), or None when it is absent."""
-    path = os.path.join(ROOT, "profiles", "walk_mixed_plummer4m_traffic.md")
     try:
-        for line in open(path):
+        for line in open(os.path.join(ROOT, TRAFFIC_PROFILE)):
             if line.startswith("traffic_bytes_per_dispatch:"):
                 return int(line.split(":")[1])
     except OSError:
@@ -77,10 +142,10 @@ def traversal_traffic():
     return None
 
 
-def energy_of(nb, state, args):
+def energy_of(nb, state, dt, theta, eps, G):
     """(E, K, U) of a state: an FP64 simulation of it, initialised (tree and
     forces), then its FP64 potential walk."""
-    s = nb.Simulation(state, args.dt, args.theta, args.softening, gravitational_constant=args.g,
+    s = nb.Simulation(state, dt, theta, eps, gravitational_constant=G,
                       precision=nb.PRECISION_FP64, bits_per_axis=21)
     s.initialize()
     k, u = s.energy()
@@ -88,25 +153,288 @@ def energy_of(nb, state, args):
     return k + u, k, u
 
 
-def accuracy_at(nb, torch, p, theta, args, targets, exact):
-    """Mixed vs FP64 Barnes-Hut and vs the direct sum at one theta, with the
-    traversal time of each precision (one force evaluation)."""
+def direct_reference(nb, torch, p, targets, eps, G):
+    dev = [torch.from_numpy(p[k]).cuda() for k in ("m", "x", "y", "z")]
+    out = [t.cpu().numpy() for t in
+           nb.direct_sum(*dev, torch.from_numpy(targets.view(np.int32)).cuda(), eps, G)]
+    del dev
+    return out
+
+
+def sample_targets(n, count, seed=0):
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(n, min(count, n), replace=False)).astype(np.uint32)
+
+
+def drift_run(nb, torch, p, theta, args, steps, targets=None, exact=None):
+    """A mixed simulation at `theta`: force accuracy at t = 0 (vs the FP64
+    pipeline and the direct sum), then `steps` steps with energy and
+    momentum drift."""
+    common = dict(gravitational_constant=args.g, bits_per_axis=21)
+    mixed = nb.Simulation(p, args.dt, theta, args.softening, precision=nb.PRECISION_MIXED,
+                          **common)
+    mixed.initialize()
+    a_mixed = mixed.accelerations()
+    f64 = nb.Simulation(p, args.dt, theta, args.softening, precision=nb.PRECISION_FP64, **common)
+    f64.initialize()
+    a64 = f64.accelerations()
+    f64.close()
     out = {}
+    out["rms_vs_fp64_bh"], out["max_vs_fp64_bh"] = map(g4, rel_err(a_mixed, a64))
+    if exact is not None:
+        out["rms_vs_direct"], out["max_vs_direct"] = map(
+            g4, rel_err([a[targets] for a in a_mixed], exact))
+        out["fp64_rms_vs_direct"], out["fp64_max_vs_direct"] = map(
+            g4, rel_err([a[targets] for a in a64], exact))
+    s0 = mixed.state()
+    e0 = energy_of(nb, s0, args.dt, theta, args.softening, args.g)
+    secs = time_steps(torch, mixed, steps)
+    s1 = mixed.state()
+    e1 = energy_of(nb, s1, args.dt, theta, args.softening, args.g)
+    mixed.close()
+    out.update({"steps": steps, "ms_per_step": round(1e3 * secs / steps, 3),
+                "energy_rel_drift": g4((e1[0] - e0[0]) / abs(e0[0])),
+                "momentum_drift": g4(float(np.abs(momentum(s1) - momentum(s0)).max()))})
+    return out
+
+
+# ------------------------------------------------------------------ legs
+def headline(nb, torch, args):
+    n = args.n
+    p = nb.generate(args.workload, n, 42)
+    if args.particle_mass is not None:
+        p["m"][:] = args.particle_mass
+    common = dict(gravitational_constant=args.g, bits_per_axis=21)
+    targets = sample_targets(n, args.direct_targets)
+    exact = direct_reference(nb, torch, p, targets, args.softening, args.g)
+
+    mixed = nb.Simulation(p, args.dt, args.theta, args.softening, precision=nb.PRECISION_MIXED,
+                          **common)
+    mixed.initialize()
+    a_mixed0 = mixed.accelerations()
+    pp, pn, visits = mixed.count_interactions(with_visits=True)
+    interactions = pp + pn
     acc = {}
+    s0 = mixed.state()
+    e0 = None if args.no_energy else energy_of(nb, s0, args.dt, args.theta, args.softening, args.g)
+    for _ in range(args.warmup):
+        mixed.step()
+    secs = time_steps(torch, mixed, args.steps)
+    phases = phase_steps(mixed)
+    # further steps to the drift span (50 steps in all by default)
+    extra = max(0, args.drift_steps - args.warmup - args.steps - 2)
+    for _ in range(extra):
+        mixed.step()
+    s1 = mixed.state()
+    e1 = None if args.no_energy else energy_of(nb, s1, args.dt, args.theta, args.softening, args.g)
+    g, m = sizes(mixed)
+    ms_per_step = 1e3 * secs / args.steps
+    value = n * args.steps / secs / 1e6
+
+    fp64 = None
+    if not args.no_fp64:
+        f64 = nb.Simulation(p, args.dt, args.theta, args.softening, precision=nb.PRECISION_FP64,
+                            **common)
+        f64.initialize()
+        a64 = f64.accelerations()
+        pp64, pn64, visits64 = f64.count_interactions(with_visits=True)
+        acc["rms_vs_fp64_bh"], acc["max_vs_fp64_bh"] = rel_err(a_mixed0, a64)
+        acc["fp64_rms_vs_direct"], acc["fp64_max_vs_direct"] = rel_err(
+            [a[targets] for a in a64], exact)
+        f64.step()  # warm-up
+        secs64 = time_steps(torch, f64, args.fp64_steps)
+        phases64 = phase_steps(f64, 1)
+        f64.close()
+        fp64 = {"value": round(n * args.fp64_steps / secs64 / 1e6, 3),
+                "ms_per_step": round(1e3 * secs64 / args.fp64_steps, 3),
+                "steps": args.fp64_steps, "phase_ms": phases64}
+        t64 = phases64["traversal"]
+        fp64["interactions_per_step"] = pp64 + pn64
+        fp64["packet_efficiency"] = round((pp64 + pn64) / (64.0 * visits64), 4) if visits64 else None
+        a64_tf = FLOPS_PER_INTERACTION * (pp64 + pn64) / (t64 * 1e-3) / 1e12
+        fp64["roofline"] = {"bound": "valu", "achieved": round(a64_tf, 3),
+                            "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": round(a64_tf / FP64_VALU_PEAK_TFLOPS, 4),
+                            "kernel": "walk_fp64_kernel"}
+    acc["rms_vs_direct"], acc["max_vs_direct"] = rel_err([a[targets] for a in a_mixed0], exact)
+
+    log("headline FP64 leg done")
+    by_theta = {}
+    if args.workload == "plummer" and args.thetas:
+        for th in (float(t) for t in args.thetas.split(",")):
+            log(f"theta {th} accuracy and drift")
+            by_theta[str(th)] = drift_run(nb, torch, p, th, args, args.drift_steps, targets,
+                                          exact)
+
+    frames = frame_capture_timing(nb, torch, mixed, n)
+    mixed.close()
+
+    trav_ms = phases["traversal"]
+    achieved = FLOPS_PER_INTERACTION * interactions / (trav_ms * 1e-3) / 1e12
+    traffic = traversal_traffic() if (args.workload == "plummer" and n == 4_194_304 and
+                                      args.theta == 0.5) else None
+    line = {
+        "metric": "Mparticle-steps/s, N=4M theta=0.5: mixed-FP32 vs FP64 at <=1e-3 rms accel err",
+        "value": round(value, 3),
+        "unit": "Mparticle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 traversal / f64 build+integrate",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"{args.workload} N={n} theta={args.theta} "
+                               f"eps={args.softening} dt={args.dt} G={args.g:g}"
+                               + (f" m={args.particle_mass:g}" if args.particle_mass else "")
+                               + " 63-bit keys",
+                   "n_particles": n, "theta": args.theta, "softening": args.softening,
+                   "dt": args.dt, "precision": "mixed"},
+        "steps_per_s": round(1e3 / ms_per_step, 4),
+        "interactions_per_step": interactions,
+        "interactions_pp_pn": [pp, pn],
+        "interactions_per_s": interactions * 1e3 / ms_per_step,
+        "packet_efficiency": round(interactions / (64.0 * visits), 4) if visits else None,
+        "groups": g, "octree_nodes": m,
+        "fp64": fp64,
+        "fp64_value": None if fp64 is None else fp64["value"],
+        "speedup_vs_fp64": None if fp64 is None else round(value / fp64["value"], 3),
+        "accuracy": {k: g4(v) for k, v in acc.items()},
+        "momentum_drift": g4(float(np.abs(momentum(s1) - momentum(s0)).max())),
+        "energy": None if e0 is None else {
+            "E_start": e0[0], "E_end": e1[0], "kinetic_start": e0[1],
+            "potential_start": e0[2], "rel_drift": g4((e1[0] - e0[0]) / abs(e0[0])),
+            "steps": args.warmup + args.steps + 2 + extra},
+        "accuracy_by_theta": by_theta,
+        "phase_ms": phases,
+        "phase_bandwidth": phase_bandwidth(phases, n, g, m),
+        "frame_capture": frames,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),
+                     "traffic": traffic,
+                     "traffic_source": (f"profile-sourced: {TRAFFIC_PROFILE} (rocprofv3 "
+                                        "FETCH_SIZE + WRITE_SIZE of this kernel on this "
+                                        "workload, calibrated against a 1 GiB copy)")
+                     if traffic else None,
+                     "kernel": "walk_mixed_kernel",
+                     "flops_per_interaction": FLOPS_PER_INTERACTION,
+                     "frac_of_non_packed_fp32": round(achieved / 78.6, 4)},
+    }
+    return line, p
+
+
+def config_uniform_1m(nb, torch, args):
+    """1,048,576 uniform in [-1,1]^3, lognormal masses, FP64, theta 0.5,
+    eps 1e-3, dt 1e-4, 20 steps; tree vs the oracle; 4096 direct-sum
+    targets; the mixed pipeline on the same workload for the ratio."""
+    import oracle
+    n, steps = 1_048_576, 20
+    p = nb.generate("uniform", n, 43)
+    common = dict(gravitational_constant=1.0, bits_per_axis=21)
+    targets = sample_targets(n, 4096, seed=1)
+    exact = direct_reference(nb, torch, p, targets, 1e-3, 1.0)
+    out = {"workload": "uniform [-1,1]^3 N=1048576 lognormal masses theta=0.5 eps=1e-3 "
+                       "dt=1e-4 20 steps 63-bit keys"}
+    for name, prec in (("fp64", nb.PRECISION_FP64), ("mixed", nb.PRECISION_MIXED)):
+        sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, precision=prec, **common)
+        sim.initialize()
+        r, mx = rel_err([a[targets] for a in sim.accelerations()], exact)
+        leg = {"rms_vs_direct": g4(r), "max_vs_direct": g4(mx)}
+        if prec == nb.PRECISION_FP64:
+            ref = oracle.Sim(p, 1e-4, 0.5, 1e-3, 1.0, 21)
+            ref.rebuild()
+            g, o = sim.structure_host(), ref.structure()
+            leg["tree_bit_exact_vs_oracle"] = bool(
+                np.array_equal(g["keys"], o["keys"]) and
+                np.array_equal(g["indices"], o["indices"]) and
+                np.array_equal(g["children"], o["children"]) and
+                np.array_equal(g["first"], o["first"]) and
+                np.array_equal(g["count"], o["count"]))
+            del ref
+        secs = time_steps(torch, sim, steps)
+        leg.update({"steps": steps, "ms_per_step": round(1e3 * secs / steps, 3),
+                    "Mparticle_steps_per_s": round(n * steps / secs / 1e6, 3)})
+        leg["phase_ms"] = phase_steps(sim, 1)
+        sim.close()
+        out[name] = leg
+    out["mixed_over_fp64"] = round(out["mixed"]["Mparticle_steps_per_s"] /
+                                   out["fp64"]["Mparticle_steps_per_s"], 3)
+    return out
+
+
+def config_clumps_16m(nb, torch, args):
+    """16,777,216 equal masses in 256 Gaussian clumps, cold start, mixed,
+    theta 0.6, eps 1e-3, dt 5e-5, 20 steps: phase breakdown with per-phase
+    HBM GB/s, traversal FLOP/s, accuracy vs the FP64 pipeline and the direct
+    sum, energy drift; the FP64 pipeline's rate for the ratio."""
+    n, steps, theta, eps, dt = 16_777_216, 20, 0.6, 1e-3, 5e-5
+    p = nb.generate("clumps", n, 44)
+    common = dict(gravitational_constant=1.0, bits_per_axis=21)
+    targets = sample_targets(n, 8192, seed=2)
+    exact = direct_reference(nb, torch, p, targets, eps, 1.0)
+    mixed = nb.Simulation(p, dt, theta, eps, precision=nb.PRECISION_MIXED, **common)
+    mixed.initialize()
+    a_mixed = mixed.accelerations()
+    pp, pn = mixed.count_interactions()
+    s0 = mixed.state()
+    e0 = energy_of(nb, s0, dt, theta, eps, 1.0)
+    mixed.step()
+    secs = time_steps(torch, mixed, steps)
+    phases = phase_steps(mixed, 2)
+    s1 = mixed.state()
+    e1 = energy_of(nb, s1, dt, theta, eps, 1.0)
+    g, m = sizes(mixed)
+    mixed.close()
+    f64 = nb.Simulation(p, dt, theta, eps, precision=nb.PRECISION_FP64, **common)
+    f64.initialize()
+    a64 = f64.accelerations()
+    secs64 = time_steps(torch, f64, 3)
+    f64.close()
+    trav = phases["traversal"]
+    tf = FLOPS_PER_INTERACTION * (pp + pn) / (trav * 1e-3) / 1e12
+    r_bh, mx_bh = rel_err(a_mixed, a64)
+    r_d, mx_d = rel_err([a[targets] for a in a_mixed], exact)
+    return {
+        "workload": "256 Gaussian clumps N=16777216 cold start theta=0.6 eps=1e-3 dt=5e-5 "
+                    "20 steps 63-bit keys",
+        "steps": steps, "ms_per_step": round(1e3 * secs / steps, 3),
+        "Mparticle_steps_per_s": round(n * steps / secs / 1e6, 3),
+        "fp64_Mparticle_steps_per_s": round(n * 3 / secs64 / 1e6, 3),
+        "mixed_over_fp64": round((n * steps / secs) / (n * 3 / secs64), 3),
+        "groups": g, "octree_nodes": m, "interactions_per_step": pp + pn,
+        "phase_ms": phases, "phase_bandwidth": phase_bandwidth(phases, n, g, m),
+        "traversal_TFLOP_per_s": round(tf, 3),
+        "traversal_frac_of_fp32_peak": round(tf / FP32_VALU_PEAK_TFLOPS, 4),
+        "rms_vs_fp64_bh": g4(r_bh), "max_vs_fp64_bh": g4(mx_bh),
+        "rms_vs_direct": g4(r_d), "max_vs_direct": g4(mx_d),
+        "energy_rel_drift": g4((e1[0] - e0[0]) / abs(e0[0])),
+        "momentum_drift": g4(float(np.abs(momentum(s1) - momentum(s0)).max())),
+    }
+
+
+def config_reference_blob(nb, torch, n):
+    """The reference's own benchmark (README.md:571-588): an N(0,1)^3 blob,
+    velocities sigma 0.01, masses 1e30 kg, SI G, theta 0.5, eps 1e-6,
+    dt 1e-3, 10 steps after initialisation; published 249 ms (100K) and
+    798 ms (250K) per FP64 step on an RTX 3070."""
+    p = nb.generate("blob", n, 42)
+    p["m"][:] = 1e30
+    out = {}
     for name, prec in (("mixed", nb.PRECISION_MIXED), ("fp64", nb.PRECISION_FP64)):
-        s = nb.Simulation(p, args.dt, theta, args.softening, gravitational_constant=args.g,
-                          precision=prec, bits_per_axis=21)
-        s.set_timing(True)
-        s.initialize()
-        acc[name] = s.accelerations()
-        out[f"{name}_traversal_ms"] = round(s.phase_times()["traversal"], 3)
-        s.close()
-    r, mx = rel_err(acc["mixed"], acc["fp64"])
-    out["rms_vs_fp64_bh"], out["max_vs_fp64_bh"] = float(f"{r:.4g}"), float(f"{mx:.4g}")
-    r, mx = rel_err([a[targets] for a in acc["mixed"]], exact)
-    out["rms_vs_direct"], out["max_vs_direct"] = float(f"{r:.4g}"), float(f"{mx:.4g}")
-    r, _ = rel_err([a[targets] for a in acc["fp64"]], exact)
-    out["fp64_rms_vs_direct"] = float(f"{r:.4g}")
+        sim = nb.Simulation(p, 1e-3, 0.5, 1e-6, gravitational_constant=6.6743e-11,
+                            precision=prec, bits_per_axis=21)
+        sim.initialize()
+        secs = time_steps(torch, sim, 10)
+        out[f"{name}_ms_per_step"] = round(1e3 * secs / 10, 3)
+        if prec == nb.PRECISION_MIXED:
+            out["mixed_phase_ms"] = phase_steps(sim, 2)
+        sim.close()
+    ref = {100_000: 249.0, 250_000: 798.0}.get(n)
+    if ref:
+        out["reference_fp64_ms_per_step_rtx3070"] = ref
     return out
 
 
@@ -142,54 +470,98 @@ def frame_capture_timing(nb, torch, sim, n, reps=20):
     return out
 
 
+# ----------------------------------------------------------------- CPU
+def host_description():
+    model = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
 def cpu_baseline(p, args):
-    """The oracle (the reference algorithm in FP64 C++/OpenMP) on the same
-    particles: one full rebuild plus the forces of a bounded sample of
-    Morton-ordered targets, scaled to one step."""
+    log("CPU baseline")
+    return _cpu_baseline(p, args)
+
+
+def _cpu_baseline(p, args):
+    """The oracle (the reference algorithm in FP64 C++/OpenMP) on the host,
+    median of 3 repeats each: (1) the headline workload, one full rebuild
+    plus the forces of a bounded sample of Morton-ordered targets,
+    extrapolated linearly to all N targets (stated in `sample`); (2) one
+    full step of the 1M uniform workload (rebuild + every target), measured
+    whole; (3) the plain CPU case end to end (Plummer 16,384, 10 steps)."""
     import oracle
+    import nbody_amd as nb
     n = len(p["m"])
     sample = min(args.cpu_targets, n)
-    sim = oracle.Sim(p, args.dt, args.theta, args.softening, args.g, 21)
-    t0 = time.perf_counter()
-    sim.rebuild()
-    t_build = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    sim.partial_forces(0, sample)
-    t_pf = time.perf_counter() - t0
-    t_forces = max(t_pf - t_build, 1e-9)
-    step = t_build + t_forces * n / sample
-    return {"value": n / step / 1e6, "unit": "Mparticle-steps/s",
+    runs = []
+    for _ in range(3):
+        sim = oracle.Sim(p, args.dt, args.theta, args.softening, args.g, 21)
+        t0 = time.perf_counter()
+        sim.rebuild()
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sim.partial_forces(0, sample)
+        t_pf = time.perf_counter() - t0
+        t_forces = max(t_pf - t_build, 1e-9)
+        runs.append((t_build + t_forces * n / sample, t_build, t_forces))
+        del sim
+    step, t_build, t_forces = sorted(runs)[1]
+    # one whole step of the 1M uniform workload
+    pu = nb.generate("uniform", 1_048_576, 43)
+    whole = []
+    for _ in range(3):
+        sim = oracle.Sim(pu, 1e-4, 0.5, 1e-3, 1.0, 21)
+        sim.initialize()
+        t0 = time.perf_counter()
+        sim.step(1)
+        whole.append(time.perf_counter() - t0)
+        del sim
+    t_1m = statistics.median(whole)
+    return {"value": round(n / step / 1e6, 4), "unit": "Mparticle-steps/s",
             "cores": oracle.threads(), "kind": "port",
-            "sample": f"one rebuild at N={n} ({t_build:.2f} s) + forces of {sample} "
-                      f"Morton-ordered targets ({t_forces:.2f} s), scaled to N",
+            "host": host_description(), "omp_threads": oracle.threads(), "repeats": 3,
+            "statistic": "median",
+            "sample": (f"headline workload: one rebuild at N={n} ({t_build:.2f} s) + forces "
+                       f"of {sample} Morton-ordered targets ({t_forces:.2f} s); the forces are "
+                       f"extrapolated linearly from {sample} to {n} targets"),
+            "uniform_1m_full_step": {"s_per_step": round(t_1m, 3),
+                                     "Mparticle_steps_per_s": round(1_048_576 / t_1m / 1e6, 4),
+                                     "measured": "whole step, every target, not extrapolated"},
             "small_plummer_end_to_end": small_plummer_end_to_end()}
 
 
 def small_plummer_end_to_end():
     """The plain CPU case end to end: Plummer N=16384, G=1, theta=0.5,
-    eps=0.01, dt=1e-3, 10 KDK steps, 30-bit keys, on the oracle and on the
-    GPU pipeline (whose FP64 states must equal the oracle's bit for bit)."""
+    eps=0.01, dt=1e-3, 10 KDK steps, 30-bit keys, on the oracle (median of 3)
+    and on the GPU pipeline (whose FP64 states must equal the oracle's bit for
+    bit)."""
     import oracle
     import nbody_amd as nb
+    import torch
     n, steps = 16384, 10
     p = nb.generate("plummer", n, 42)
-    ref = oracle.Sim(p, 1e-3, 0.5, 0.01, 1.0, 10)
-    ref.initialize()
-    t0 = time.perf_counter()
-    ref.step(steps)
-    t_cpu = time.perf_counter() - t0
-    out = {"n": n, "steps": steps, "cpu_ms_per_step": round(1e3 * t_cpu / steps, 3)}
+    times = []
+    for _ in range(3):
+        ref = oracle.Sim(p, 1e-3, 0.5, 0.01, 1.0, 10)
+        ref.initialize()
+        t0 = time.perf_counter()
+        ref.step(steps)
+        times.append(time.perf_counter() - t0)
+    out = {"n": n, "steps": steps,
+           "cpu_ms_per_step": round(1e3 * statistics.median(times) / steps, 3)}
     rs = ref.state()
     for name, prec in (("fp64", nb.PRECISION_FP64), ("mixed", nb.PRECISION_MIXED)):
         sim = nb.Simulation(p, 1e-3, 0.5, 0.01, gravitational_constant=1.0,
                             precision=prec, bits_per_axis=10)
         sim.initialize()
-        import torch
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sim.run(steps)
-        torch.cuda.synchronize()
-        out[f"gpu_{name}_ms_per_step"] = round(1e3 * (time.perf_counter() - t0) / steps, 3)
+        secs = time_steps(torch, sim, steps)
+        out[f"gpu_{name}_ms_per_step"] = round(1e3 * secs / steps, 3)
         st = sim.state()
         if prec == nb.PRECISION_FP64:
             out["fp64_state_bit_exact"] = all(np.array_equal(st[f], rs[f])
@@ -201,7 +573,8 @@ def small_plummer_end_to_end():
     return out
 
 
-def adjoint_workload(args):
+# ------------------------------------------------------------- adjoint
+def adjoint_workload(args, probes=None, quiet=False):
     """The differentiable workload: K leapfrog steps of the FP64 pipeline on
     the disk + bulge galaxy, reverse mode with checkpoints, and the gradient
     of L = sum_i |r_i(T) - r*_i|^2 (r* a perturbed copy of the final
@@ -210,10 +583,10 @@ def adjoint_workload(args):
     import torch
     import nbody_amd as nb
 
-    n, K, every = args.n, args.adjoint_steps, args.adjoint_every
+    n, K, every = args.adjoint_n, args.adjoint_steps, args.adjoint_every
+    nprobe = args.adjoint_probes if probes is None else probes
     p = nb.generate("disk", n, 42)
-    adj = nb.Adjoint(n, args.dt, args.theta, args.softening, K, every,
-                     gravitational_constant=args.g)
+    adj = nb.Adjoint(n, 1e-3, 0.5, 0.01, K, every, gravitational_constant=1.0)
     state = [torch.from_numpy(p[k]).cuda() for k in ("x", "y", "z", "vx", "vy", "vz")]
     m = torch.from_numpy(p["m"]).cuda()
 
@@ -246,8 +619,8 @@ def adjoint_workload(args):
     gm = grads[6].cpu().numpy()
 
     rng = np.random.default_rng(2)
-    probes = []
-    for k in range(args.adjoint_probes):
+    results = []
+    for k in range(nprobe):
         i = int(rng.integers(n))
         on_mass = k % 2 == 1
         c = int(rng.integers(3))
@@ -264,12 +637,13 @@ def adjoint_workload(args):
         fd1 = (f(h) - f(-h)) / (2 * h)
         fd2 = (f(2 * h) - f(-2 * h)) / (4 * h)
         ad = float(gm[i] if on_mass else gv[c, i])
-        probes.append({"what": "mass" if on_mass else f"v{'xyz'[c]}", "index": i,
-                       "adjoint": ad, "fd": fd1,
-                       "smooth": bool(abs(fd1 - fd2) <= 1e-4 * abs(fd1)),
-                       "rel_err": abs(ad - fd1) / max(abs(fd1), 1e-300)})
+        results.append({"what": "mass" if on_mass else f"v{'xyz'[c]}", "index": i,
+                        "adjoint": ad, "fd": fd1,
+                        "smooth": bool(abs(fd1 - fd2) <= 1e-4 * abs(fd1)),
+                        "rel_err": abs(ad - fd1) / max(abs(fd1), 1e-300)})
+
     def worst(kind):
-        e = [q["rel_err"] for q in probes if q["smooth"] and (q["what"] == "mass") == kind]
+        e = [q["rel_err"] for q in results if q["smooth"] and (q["what"] == "mass") == kind]
         return (len(e), max(e) if e else None)
     fwd_ms, bwd_ms = 1e3 * (t1 - t0), 1e3 * (t3 - t2)
     line = {
@@ -277,20 +651,22 @@ def adjoint_workload(args):
         "value": round(n * K / ((t1 - t0) + (t3 - t2)) / 1e6, 3),
         "unit": "Mparticle-steps/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f64", "data": "synthetic (seeded generator)",
-        "config": {"workload": f"disk+bulge N={n} theta={args.theta} eps={args.softening} "
-                               f"dt={args.dt} K={K} checkpoints every {every}",
+        "config": {"workload": f"disk+bulge N={n} theta=0.5 eps=0.01 dt=1e-3 K={K} "
+                               f"checkpoints every {every}",
                    "n_particles": n, "steps": K, "checkpoint_every": every},
         "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
         "backward_over_forward": round(bwd_ms / fwd_ms, 3),
         "loss": "sum |r(T) - r*|^2, r* = r(T) + 1e-3 N(0,1)",
-        "fd_probes": probes,
+        "fd_probes": results,
         # probes whose steps h and 2h agree to 1e-4 (no opening decision
         # flipped within 2h), and their largest relative error
         "fd_velocity_smooth_max_rel_err": worst(False),
         "fd_mass_smooth_max_rel_err": worst(True),
     }
     adj.close()
-    print(json.dumps(line))
+    if not quiet:
+        print(json.dumps(line))
+    return line
 
 
 def main():
@@ -308,21 +684,24 @@ def main():
                     help="give every particle this mass (the reference's benchmark "
                          "uses 1e30 with --g 6.6743e-11)")
     ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--drift-steps", type=int, default=50,
+                    help="steps over which energy and momentum drift are measured")
     ap.add_argument("--direct-targets", type=int, default=8192)
-    ap.add_argument("--cpu-targets", type=int, default=524288)
+    ap.add_argument("--cpu-targets", type=int, default=131072)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (the "
                          "reference dispatch for FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--thetas", default="0.3,0.8",
-                    help="extra opening angles for the accuracy table (Plummer only)")
+                    help="extra opening angles for the accuracy and drift table (Plummer only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-energy", action="store_true",
                     help="skip the FP64 energy check (it builds a second, FP64 simulation)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="measure the headline workload only (no other configs)")
     ap.add_argument("--adjoint", action="store_true",
-                    help="measure the differentiable workload instead (FP64 forward + "
-                         "reverse over --adjoint-steps steps; use with --n 262144 "
-                         "--theta 0.5 --softening 0.01 --dt 1e-3)")
+                    help="measure only the differentiable workload (FP64 forward + reverse)")
+    ap.add_argument("--adjoint-n", type=int, default=262_144)
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--adjoint-every", type=int, default=10)
     ap.add_argument("--adjoint-probes", type=int, default=16)
@@ -347,122 +726,26 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    n = args.n
-    p = nb.generate(args.workload, n, 42)
-    if args.particle_mass is not None:
-        p["m"][:] = args.particle_mass
-    common = dict(gravitational_constant=args.g, bits_per_axis=21)
-
-    mixed = nb.Simulation(p, args.dt, args.theta, args.softening,
-                          precision=nb.PRECISION_MIXED, **common)
-    mixed.set_timing(True)
-    mixed.initialize()
-    a_mixed0 = mixed.accelerations()
-    pp, pn, visits = mixed.count_interactions(with_visits=True)
-    interactions = pp + pn
-
-    # accuracy at t = 0 against the FP64 pipeline (bit-exact with the oracle)
-    # and against an FP64 direct sum on sampled targets
-    acc = {}
-    fp64_rate = None
-    if not args.no_fp64:
-        f64 = nb.Simulation(p, args.dt, args.theta, args.softening,
-                            precision=nb.PRECISION_FP64, **common)
-        f64.set_timing(True)
-        f64.initialize()
-        a64 = f64.accelerations()
-        acc["rms_vs_fp64_bh"], acc["max_vs_fp64_bh"] = rel_err(a_mixed0, a64)
-    rng = np.random.default_rng(0)
-    targets = np.sort(rng.choice(n, min(args.direct_targets, n), replace=False)).astype(np.uint32)
-    dev = [torch.from_numpy(p[k]).cuda() for k in ("m", "x", "y", "z")]
-    exact = [t.cpu().numpy() for t in
-             nb.direct_sum(*dev, torch.from_numpy(targets.view(np.int32)).cuda(),
-                           args.softening, args.g)]
-    acc["rms_vs_direct"], acc["max_vs_direct"] = rel_err([a[targets] for a in a_mixed0], exact)
-    if not args.no_fp64:
-        acc["fp64_rms_vs_direct"], acc["fp64_max_vs_direct"] = rel_err(
-            [a[targets] for a in a64], exact)
-    del dev
-
-    s0 = mixed.state()
-    p0 = momentum(s0)
-    e0 = None if args.no_energy else energy_of(nb, s0, args)
-    for _ in range(args.warmup):
-        mixed.step()
-    secs, phases = time_steps(nb, torch, mixed, args.steps)
-    s1 = mixed.state()
-    p1 = momentum(s1)
-    e1 = None if args.no_energy else energy_of(nb, s1, args)
-    ms_per_step = 1e3 * secs / args.steps
-    value = n * args.steps / secs / 1e6
-
-    energy64 = None
-    if not args.no_fp64:
-        f64.step()  # warm-up
-        f0 = f64.energy()
-        secs64, phases64 = time_steps(nb, torch, f64, args.fp64_steps)
-        fp64_rate = n * args.fp64_steps / secs64 / 1e6
-        f1 = f64.energy()
-        energy64 = (sum(f1) - sum(f0)) / abs(sum(f0))
-        f64.close()
-
-    by_theta = {}
-    if args.workload == "plummer" and args.thetas:
-        for th in (float(t) for t in args.thetas.split(",")):
-            by_theta[str(th)] = accuracy_at(nb, torch, p, th, args, targets, exact)
-
-    frames = frame_capture_timing(nb, torch, mixed, n)
-
-    trav_ms = phases["traversal"]
-    achieved = FLOPS_PER_INTERACTION * interactions / (trav_ms * 1e-3) / 1e12
-    line = {
-        "metric": "Mparticle-steps/s, N=4M theta=0.5: mixed-FP32 vs FP64 at <=1e-3 rms accel err",
-        "value": round(value, 3),
-        "unit": "Mparticle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 traversal / f64 build+integrate",
-        "data": "synthetic (seeded generator)",
-        "config": {"workload": f"{args.workload} N={n} theta={args.theta} "
-                               f"eps={args.softening} dt={args.dt} G={args.g:g}"
-                               + (f" m={args.particle_mass:g}" if args.particle_mass else "")
-                               + " 63-bit keys",
-                   "n_particles": n, "theta": args.theta, "softening": args.softening,
-                   "dt": args.dt, "precision": "mixed"},
-        "steps_per_s": round(1e3 / ms_per_step, 4),
-        "interactions_per_step": interactions,
-        "interactions_pp_pn": [pp, pn],
-        "interactions_per_s": interactions * 1e3 / ms_per_step,
-        "packet_efficiency": round(interactions / (64.0 * visits), 4) if visits else None,
-        "fp64_value": None if fp64_rate is None else round(fp64_rate, 3),
-        "speedup_vs_fp64": None if fp64_rate is None else round(value / fp64_rate, 3),
-        "accuracy": {k: float(f"{v:.4g}") for k, v in acc.items()},
-        "momentum_drift": float(np.abs(p1 - p0).max()),
-        "energy": None if e0 is None else {
-            "E_start": e0[0], "E_end": e1[0], "kinetic_start": e0[1],
-            "potential_start": e0[2], "rel_drift": (e1[0] - e0[0]) / abs(e0[0]),
-            "steps": args.warmup + args.steps,
-            "fp64_rel_drift": energy64, "fp64_steps": args.fp64_steps},
-        "accuracy_by_theta": by_theta,
-        "phase_ms": {k: round(v, 4) for k, v in phases.items()},
-        "fp64_phase_ms": None if fp64_rate is None else
-        {k: round(v, 4) for k, v in phases64.items()},
-        "frame_capture": frames,
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
-                     "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),
-                     "traffic": traversal_traffic() if (args.workload == "plummer" and
-                                                        n == 4_194_304 and
-                                                        args.theta == 0.5) else None,
-                     "kernel": "walk_mixed_kernel",
-                     "flops_per_interaction": FLOPS_PER_INTERACTION},
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(p, args),
-    }
+    log("headline workload")
+    line, p = headline(nb, torch, args)
+    log(f"headline: {line['value']} Mparticle-steps/s, {line['ms_per_step']} ms/step")
+    if not args.no_configs:
+        cfg = {}
+        cfg["uniform_1m_fp64"] = config_uniform_1m(nb, torch, args)
+        log("uniform 1M done")
+        cfg["clumps_16m_mixed"] = config_clumps_16m(nb, torch, args)
+        log("clumps 16M done")
+        cfg["reference_blob_100k"] = config_reference_blob(nb, torch, 100_000)
+        cfg["reference_blob_250k"] = config_reference_blob(nb, torch, 250_000)
+        log("reference blobs done")
+        adj = adjoint_workload(args, probes=4, quiet=True)
+        log("adjoint done")
+        cfg["disk_bulge_adjoint"] = {k: adj[k] for k in (
+            "value", "unit", "config", "forward_ms", "backward_ms", "backward_over_forward",
+            "fd_probes", "fd_velocity_smooth_max_rel_err", "fd_mass_smooth_max_rel_err")}
+        line["configs"] = cfg
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(p, args)
+    log("done")
     print(json.dumps(line))
 
 
