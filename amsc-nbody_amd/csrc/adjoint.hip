@@ -36,7 +36,7 @@ void launch_energy_partials(hipStream_t s, const double* m, const double* vx,
                             const double* phi, size_t n, double* partials,
                             int blocks);
 void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
-                           int max_level, const int* pre_of,
+                           int max_level, const int2* pos,
                            const Walk64Rec* w64, const double4* wcenter,
                            const int* d_wcount, int wcount_host,
                            const double4* rec, const uint32_t* idx, int n,
@@ -120,7 +120,7 @@ void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scr
     const double eps = s->cfg.softening;
     sim_fill_views(s); // the push-down reads node masses
     nbh_octree_view o = s->octree_view();
-    launch_force_vjp_fp64(s->stream, o, s->cfg.bits_per_axis, s->pre.p,
+    launch_force_vjp_fp64(s->stream, o, s->cfg.bits_per_axis, s->pos.p,
                           s->w64.p, s->w64center.p, s->scalars.p + 3, static_cast<int>(M),
                           s->rec64.p, s->idx.p, static_cast<int>(n), eps * eps,
                           s->cfg.gravitational_constant, contains, abar,

@@ -85,6 +85,30 @@ __device__ __forceinline__ unsigned block256_exclusive_sum(unsigned v,
     return off + inc - v;
 }
 
+// Adds one per active lane to the LDS counter cnt[key], one atomic per
+// distinct key in the wave; returns each active lane's slot (the counter's
+// value before its own increment, lanes of one key in lane order).  Every
+// lane of the wave must call it (inactive lanes with active = false).
+__device__ __forceinline__ int wave_claim(int* cnt, int key, bool active)
+{
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    int slot = 0;
+    unsigned long long todo = __ballot(active);
+    while (todo != 0ull) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const int k0 = __shfl(key, leader);
+        const unsigned long long same = __ballot(active && key == k0);
+        int base = 0;
+        if (lane == leader)
+            base = atomicAdd(&cnt[k0], __popcll(same));
+        base = __shfl(base, leader);
+        if (active && key == k0)
+            slot = base + __popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return slot;
+}
+
 // Agent-scope atomics for cross-workgroup protocols (look-back scans,
 // arrival counters).  gfx950 has one L2 per XCD: agent scope makes these
 // coherent across all eight.

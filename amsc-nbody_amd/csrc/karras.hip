@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void karras_kernel(
     const K* __restrict__ keys, const int* __restrict__ d_n, int n_host,
     int* __restrict__ left, int* __restrict__ right, int* __restrict__ parent,
     int* __restrict__ range_first, int* __restrict__ range_last,
-    int* __restrict__ prefix_len)
+    int* __restrict__ prefix_len, int* __restrict__ parent_prefix_len)
 {
     const int n = d_n ? *d_n : n_host;
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -88,6 +88,12 @@ __global__ __launch_bounds__(256) void karras_kernel(
     right[i] = rc;
     parent[lc] = n + i;
     parent[rc] = n + i;
+    if (parent_prefix_len) {
+        // each child's parent prefix length, so that the radix -> octree
+        // plan reads its parent's level sequentially
+        parent_prefix_len[lc] = node_prefix;
+        parent_prefix_len[rc] = node_prefix;
+    }
 }
 
 // tree_builder.cu:115-180: leaf mass and centre of mass; one particle per
@@ -173,19 +179,20 @@ __global__ __launch_bounds__(256) void tree_climb_kernel(nbh_tree_view t)
 template <typename K>
 void launch_karras(hipStream_t s, const K* keys, const int* d_nleaves,
                    int max_leaves, int* left, int* right, int* parent,
-                   int* range_first, int* range_last, int* prefix_len)
+                   int* range_first, int* range_last, int* prefix_len,
+                   int* parent_prefix_len)
 {
     const int threads = max_leaves > 1 ? max_leaves - 1 : 1;
     karras_kernel<K><<<blocks_for(threads, 256), 256, 0, s>>>(
         keys, d_nleaves, max_leaves, left, right, parent, range_first,
-        range_last, prefix_len);
+        range_last, prefix_len, parent_prefix_len);
     NBH_HIP(hipGetLastError());
 }
 
 template void launch_karras<uint32_t>(hipStream_t, const uint32_t*, const int*, int, int*,
-                                      int*, int*, int*, int*, int*);
+                                      int*, int*, int*, int*, int*, int*);
 template void launch_karras<uint64_t>(hipStream_t, const uint64_t*, const int*, int, int*,
-                                      int*, int*, int*, int*, int*);
+                                      int*, int*, int*, int*, int*, int*);
 
 void launch_tree_com(hipStream_t s, const nbh_tree_view& t, const int* first,
                      const int* count, const uint32_t* idx, const double* m,

@@ -195,6 +195,12 @@ void launch_radix_passes(hipStream_t s, K* keys, uint32_t* vals, K* keys_alt,
                          uint32_t* vals_alt, size_t n, int passes,
                          uint32_t* hist, uint32_t* status);
 
+// Octree levels the pipeline's level passes handle (63-bit keys give 22),
+// and the spacing of the per-level internal-node counters (one 128-byte
+// line each, so that tiles adding to different levels do not share a line).
+constexpr int kMaxLevels = 32;
+constexpr int kLevelCountStride = 32;
+
 // scan.hip
 size_t scan_status_words(size_t n);
 template <typename K>
@@ -206,13 +212,15 @@ void launch_plan_scan(hipStream_t s, const int* d_nleaves, int max_leaves,
                       const int* tparent, const int* prefix_len,
                       const typename Layout<BITS>::Key* unique, int* level,
                       int* count, int* offset, int* d_noctree, int* err,
-                      unsigned long long* status);
+                      unsigned long long* status, int* level_count = nullptr,
+                      const int* parent_prefix_len = nullptr);
 
 // karras.hip
 template <typename K>
 void launch_karras(hipStream_t s, const K* keys, const int* d_nleaves,
                    int max_leaves, int* left, int* right, int* parent,
-                   int* range_first, int* range_last, int* prefix_len);
+                   int* range_first, int* range_last, int* prefix_len,
+                   int* parent_prefix_len = nullptr);
 void launch_tree_com(hipStream_t s, const nbh_tree_view& t, const int* first,
                      const int* count, const uint32_t* idx, const double* m,
                      const double* x, const double* y, const double* z);
@@ -225,7 +233,9 @@ void launch_octree_build(hipStream_t s, const nbh_octree_view& o,
                          const typename Layout<BITS>::Key* unique,
                          const int* gfirst, const int* gcount,
                          const int* plevel, const int* pcount,
-                         const int* poffset, int* err, bool count_children);
+                         const int* poffset, int* err, bool count_children,
+                         const double4* rec64 = nullptr, double4* sum4 = nullptr,
+                         int4* aux = nullptr);
 template <int BITS>
 void launch_geometry(hipStream_t s, const nbh_octree_view& o, double cx,
                      double cy, double cz, double side, int* err);
@@ -257,6 +267,28 @@ struct alignas(64) Walk64Rec {
     double gm;
 };
 static_assert(sizeof(Walk64Rec) == 64, "FP64 walk records are 64 bytes");
+
+// Per-node state of the pipeline's level passes (one entry per octree
+// node in each array):
+//   sum4  {com x, y, z, mass}        written bottom-up by the node
+//   aux   {size, hcount, pc, 0}      bottom-up by the node: walk positions
+//                                    of the subtree, its particle count, a
+//                                    leaf's own particle count (0 internal)
+//   crit  opening radius             bottom-up by an internal node
+//   link  {local, hoffset}           bottom-up by the parent: position and
+//                                    Hilbert-curve offsets inside the
+//                                    parent's subtree
+//   pos   {pre, hoffset}             top-down by the node: absolute
+//                                    pre-order position and curve offset
+// A parent reads each child's sum4 and aux; the top-down passes read the
+// node's own words and the parent's pos.
+struct LevelArrays {
+    double4* sum4;
+    int4* aux;
+    double* crit;
+    int2* link;
+    int2* pos;
+};
 
 // walk_layout.hip, walk_fp64.hip, walk_mixed.hip
 void launch_generic_sizes(hipStream_t s, const nbh_octree_view& o, int* size,

@@ -27,6 +27,7 @@ constexpr int kSortThreads = 256;
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagInclusive = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
+constexpr int kLookback = 4; // predecessor statuses read per round trip
 
 template <typename K>
 struct SortTile;
@@ -166,18 +167,38 @@ __global__ __launch_bounds__(256) void onesweep_kernel(
 
     uint32_t prefix = 0;
     if (tile > 0) {
+        // Look back kLookback tiles per round trip: the statuses of tiles
+        // j, j-1, ... are read together and consumed in order up to the
+        // first inclusive one (done) or the first one not yet published
+        // (read again from there).  Tiles below 0 read as inclusive zeros.
         int j = static_cast<int>(tile) - 1;
-        while (true) {
-            const uint32_t sw = load_agent(status + static_cast<size_t>(j) * 256 + d);
-            const uint32_t flag = sw & ~kValueMask;
-            if (flag == 0u) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+        bool done = false;
+        while (!done) {
+            uint32_t sw[kLookback];
+#pragma unroll
+            for (int q = 0; q < kLookback; ++q)
+                sw[q] = (j - q >= 0)
+                            ? load_agent(status + static_cast<size_t>(j - q) * 256 + d)
+                            : kFlagInclusive;
+            int used = 0;
+            bool stalled = false;
+#pragma unroll
+            for (int q = 0; q < kLookback; ++q) {
+                if (done || stalled)
+                    continue;
+                const uint32_t flag = sw[q] & ~kValueMask;
+                if (flag == 0u) {
+                    stalled = true;
+                    continue;
+                }
+                prefix += sw[q] & kValueMask;
+                ++used;
+                if (flag == kFlagInclusive)
+                    done = true;
             }
-            prefix += sw & kValueMask;
-            if (flag == kFlagInclusive)
-                break;
-            --j;
+            j -= used;
+            if (!done && used == 0)
+                __builtin_amdgcn_s_sleep(1);
         }
         store_agent(my_status, kFlagInclusive | (prefix + count));
     }

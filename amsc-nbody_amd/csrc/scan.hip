@@ -25,7 +25,11 @@ namespace nbh {
 
 namespace {
 
-constexpr int kScanThreads = 256;
+// 8192-element tiles: the inclusive prefix travels back from the first
+// tile by about 64 tiles per look-back round trip, so the tile count bounds
+// the scan's latency (2048-element tiles: about 2x slower at 8M elements).
+constexpr int kScanThreads = 1024;
+constexpr int kScanWaves = kScanThreads / 64;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr unsigned long long kAgg = 1ull << 62;
@@ -88,7 +92,7 @@ template <typename T>
 __device__ void tile_scan(T (&v)[kScanItems], T identity, uint32_t tile,
                           unsigned long long* status)
 {
-    __shared__ T s_wave[4];
+    __shared__ T s_wave[kScanWaves];
     __shared__ T s_prefix;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -111,7 +115,7 @@ __device__ void tile_scan(T (&v)[kScanItems], T identity, uint32_t tile,
     T wave_prefix = identity;
     T tile_total = identity;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
+    for (int w = 0; w < kScanWaves; ++w) {
         if (w < wave)
             wave_prefix = combine(wave_prefix, s_wave[w]);
         tile_total = combine(tile_total, s_wave[w]);
@@ -227,12 +231,16 @@ __global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(
     const typename Layout<BITS>::Key* __restrict__ unique,
     int* __restrict__ level, int* __restrict__ count, int* __restrict__ offset,
     int* __restrict__ d_noctree, int* err, unsigned long long* status,
-    unsigned* ticket)
+    unsigned* ticket, int* __restrict__ level_count,
+    const int* __restrict__ parent_plen)
 {
     using L = Layout<BITS>;
     __shared__ uint32_t s_tile;
+    __shared__ int s_lc[kMaxLevels];
     if (threadIdx.x == 0)
         s_tile = atomicAdd(ticket, 1u);
+    if (threadIdx.x < kMaxLevels)
+        s_lc[threadIdx.x] = 0;
     __syncthreads();
     const int nl = *d_nleaves;
     const int R = 2 * nl - 1;
@@ -275,7 +283,11 @@ __global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(
                     raise_error(err, 2);
                     ok = false;
                 } else {
-                    parent_level = level_of(p);
+                    // the parent's prefix length, from the Karras build when
+                    // it recorded one per child (a sequential read)
+                    parent_level = parent_plen
+                                       ? (parent_plen[node] - L::padding_bits) / 3
+                                       : level_of(p);
                 }
             }
         }
@@ -293,7 +305,25 @@ __global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j)
         c[j] = v[j];
+    // Internal nodes per octree level: internal radix node r owns the chain
+    // of octree nodes at levels level[r] - count[r] + 1 .. level[r] (a radix
+    // leaf owns one octree leaf).  Counted per tile in LDS, one global add
+    // per level and tile (the counters start at zero).
+    if (level_count) {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int node = base + j;
+            const int cj = (node >= nl && node < R) ? static_cast<int>(c[j]) : 0;
+            for (int k = 0; __ballot(k < cj) != 0ull; ++k) {
+                const int lk = lv[j] - cj + 1 + k;
+                (void)wave_claim(s_lc, lk, k < cj && lk >= 0 && lk < kMaxLevels);
+            }
+        }
+    }
     tile_scan<long long>(v, 0ll, tile, status);
+    // tile_scan's barriers order the LDS counts before these reads
+    if (level_count && threadIdx.x < kMaxLevels && s_lc[threadIdx.x] != 0)
+        atomicAdd(&level_count[threadIdx.x * kLevelCountStride], s_lc[threadIdx.x]);
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const int node = base + j;
@@ -337,14 +367,15 @@ void launch_plan_scan(hipStream_t s, const int* d_nleaves, int max_leaves,
                       const int* tparent, const int* prefix_len,
                       const typename Layout<BITS>::Key* unique, int* level,
                       int* count, int* offset, int* d_noctree, int* err,
-                      unsigned long long* status)
+                      unsigned long long* status, int* level_count,
+                      const int* parent_plen)
 {
     const size_t rmax = 2 * static_cast<size_t>(max_leaves);
     const size_t tiles = (rmax + kScanTile - 1) / kScanTile;
     unsigned* ticket = reinterpret_cast<unsigned*>(status + tiles + 32);
     plan_scan_kernel<BITS><<<static_cast<unsigned>(tiles), kScanThreads, 0, s>>>(
         d_nleaves, tparent, prefix_len, unique, level, count, offset,
-        d_noctree, err, status, ticket);
+        d_noctree, err, status, ticket, level_count, parent_plen);
     NBH_HIP(hipGetLastError());
 }
 
@@ -354,9 +385,9 @@ template void launch_leaf_groups<uint64_t>(hipStream_t, const uint64_t*, int, ui
                                            int*, int*, int*, unsigned long long*);
 template void launch_plan_scan<10>(hipStream_t, const int*, int, const int*, const int*,
                                    const uint32_t*, int*, int*, int*, int*, int*,
-                                   unsigned long long*);
+                                   unsigned long long*, int*, const int*);
 template void launch_plan_scan<21>(hipStream_t, const int*, int, const int*, const int*,
                                    const uint64_t*, int*, int*, int*, int*, int*,
-                                   unsigned long long*);
+                                   unsigned long long*, int*, const int*);
 
 } // namespace nbh

@@ -4,30 +4,43 @@
 
 #include "nbh_internal.hpp"
 
+#include <cstddef>
+
 namespace nbh {
 
 const char* traversal_message(int code);
 const char* stage_message(int word, int code);
 
+// Start of each level's internal nodes in the level-grouped node list
+// (s[L + 1] ends level L; s[kMaxLevels] is the internal node total).
+struct LevelStarts {
+    int s[kMaxLevels + 1];
+};
+// The level starts from the plan scan's per-level counts (one counter per
+// kLevelCountStride ints, the root excluded).
+LevelStarts level_starts(const int* level_count);
+// The internal nodes grouped by level: order[start[L] .. start[L + 1]) are
+// the octree nodes of level L (the root first), from the plan.  cursor:
+// kMaxLevels * kLevelCountStride zeroed ints; n_leaves the host's group
+// count.
+void launch_level_scatter(hipStream_t s, const int* d_nleaves, int n_leaves,
+                          const int* plevel, const int* pcount, const int* poffset,
+                          const int* level_count, int* cursor, int* order);
+// Mass / COM, subtree sizes and the walk layout over the level-grouped
+// internal nodes and the leaves (one per Morton group); the leaves' sums and
+// aux words come from the octree materialisation.
 template <int BITS>
-void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
-                         bool mixed, const double4* rec64, const int4* rec32,
-                         double theta, const double* box,
-                         bool need_center, double eps, int* size, int* local,
-                         double* crit, int* pre, int* order, int* level_info,
-                         double4* sum4, const int* poffset, const int* pcount,
-                         int n_groups, double G, Walk64Rec* w64, double4* w64center,
-                         int4* w32node,
-                         int4* w32link, int4* w32center, int* wcount,
-                         hipEvent_t between, int* hcount = nullptr,
-                         int* hoffset = nullptr, uint32_t* horder = nullptr,
-                         const uint8_t* htab = nullptr);
-// ints of the level-pass bookkeeping array (level starts + per-block counts)
-size_t level_info_ints();
+void launch_level_passes(hipStream_t s, const nbh_octree_view& o, const LevelStarts& st,
+                         bool mixed, const double4* rec64, const int4* rec32, double theta,
+                         const double* box, bool need_center, double eps,
+                         const LevelArrays& la, const int* order, const int* poffset,
+                         const int* pcount, int n_groups, double G, Walk64Rec* w64,
+                         double4* w64center, int4* w32node, int4* w32center, int* wcount,
+                         hipEvent_t between, uint32_t* horder, const uint8_t* htab);
 void launch_walk_sentinel(hipStream_t s, int4* w32node, Walk64Rec* w64, int cap);
 template <int BITS>
-void launch_octree_views(hipStream_t s, const nbh_octree_view& o,
-                         const double4* sum4, const double* box);
+void launch_octree_views(hipStream_t s, const nbh_octree_view& o, const LevelArrays& la,
+                         const double* box);
 
 template <typename T>
 struct DevBuf {
@@ -77,14 +90,25 @@ enum Phase : int {
 
 
 
-// Host copy of the mid-step readback (pinned).
+// The mid-step readback, one copy: the device keeps the same layout at the
+// end of its per-step cleared words (errors, level counters and scatter
+// cursors are cleared; the counts and the box are written every step), the
+// host a pinned copy.
+constexpr int kErrorWords = 64;
 struct Readback {
+    int errors[kErrorWords]; // ERR_WORD_COUNT used
+    // internal octree nodes per level (root excluded), one line per level
+    int level_count[kMaxLevels * kLevelCountStride];
+    int level_cursor[kMaxLevels * kLevelCountStride]; // level scatter cursors
     int ngroups;
     int nnodes;
-    int errors[ERR_WORD_COUNT];
-    int pad;
+    int pad[2];
     double box[5]; // centre x, y, z, side, largest |m|
 };
+constexpr size_t kReadbackClearedWords =
+    offsetof(Readback, ngroups) / sizeof(int);
+static_assert(ERR_WORD_COUNT <= kErrorWords, "error words fit the readback");
+static_assert(offsetof(Readback, box) % 8 == 0, "box is 8-byte aligned");
 
 } // namespace nbh
 
@@ -113,6 +137,7 @@ struct nbh_sim {
     nbh::DevBuf<int4> rec32;
     nbh::DevBuf<int> gfirst, gcount;
     nbh::DevBuf<int> left, right, tparent, rfirst, rlast, plen;
+    nbh::DevBuf<int> pplen; // per radix node: its parent's prefix length
     nbh::DevBuf<int> plevel, pcount, poffset;
     // per-step cleared words: hist | sort status | group scan | plan scan |
     // error words
@@ -123,10 +148,13 @@ struct nbh_sim {
     unsigned long long* group_status = nullptr;
     unsigned long long* plan_status = nullptr;
     int* errors = nullptr;
+    // the device side of the mid-step readback (inside `zero`): errors,
+    // group and node counts, level starts, box
+    nbh::Readback* mid = nullptr;
+    double* box = nullptr;        // mid->box: centre x, y, z, side, largest |m|
     // small device scalars
-    nbh::DevBuf<double> box;      // centre x, y, z, side, largest |m|
     nbh::DevBuf<double> partials; // bbox partials
-    nbh::DevBuf<int> scalars;     // [1] ngroups [2] nnodes [3] wcount ([0] unused)
+    nbh::DevBuf<int> scalars;     // [3] wcount ([0..2] unused)
     nbh::DevBuf<unsigned long long> counters;
     nbh::DevBuf<int> walkq;       // mixed traversal work-queue counters
     // mixed traversal split into subtree units (few packets only): the
@@ -139,16 +167,20 @@ struct nbh_sim {
     int nnodes = 0;
     int ngroups = 0;
     nbh::DevBuf<int> children, oparent, level, ofirst, ocount, pending,
-        child_count, size, local, pre, order, level_info;
+        child_count, order;
     nbh::DevBuf<unsigned char> prefix;
-    nbh::DevBuf<double> mass, comx, comy, comz, cenx, ceny, cenz, half, crit;
+    nbh::DevBuf<double> mass, comx, comy, comz, cenx, ceny, cenz, half;
     nbh::DevBuf<nbh::Walk64Rec> w64; // FP64 walk records
     nbh::DevBuf<double4> w64center;
-    nbh::DevBuf<double4> sum4; // packed {com, mass} per node (level passes)
-    bool views_fresh = false;  // reference-layout node arrays match sum4
+    // per-node state of the level passes
+    nbh::DevBuf<double4> sum4;
+    nbh::DevBuf<int4> aux;
+    nbh::DevBuf<double> crit;
+    nbh::DevBuf<int2> link, pos;
+    nbh::LevelStarts starts{};       // level starts of the last rebuild
+    bool views_fresh = false;        // reference-layout node arrays match sum4
     nbh::DevBuf<int4> w32node, w32center;
     // the traversal's target order along the Hilbert curve
-    nbh::DevBuf<int> hcount, hoffset;    // per node
     nbh::DevBuf<uint32_t> horder;        // per particle
     nbh::DevBuf<uint8_t> htab;           // the curve's state tables
 
@@ -165,6 +197,11 @@ struct nbh_sim {
                 (void)hipEventDestroy(e);
         if (own_stream && stream)
             (void)hipStreamDestroy(stream);
+    }
+
+    nbh::LevelArrays level_arrays()
+    {
+        return nbh::LevelArrays{sum4.p, aux.p, crit.p, link.p, pos.p};
     }
 
     nbh_octree_view octree_view()

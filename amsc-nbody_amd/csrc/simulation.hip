@@ -68,6 +68,7 @@ void alloc_state(nbh_sim* s)
     s->rlast.alloc(n);
     s->plen.alloc(n);
     s->tparent.alloc(2 * n);
+    s->pplen.alloc(2 * n);
     s->plevel.alloc(2 * n);
     s->pcount.alloc(2 * n);
     s->poffset.alloc(2 * n);
@@ -75,23 +76,25 @@ void alloc_state(nbh_sim* s)
     const size_t sort_w = sort_status_words(n, s->passes);
     const size_t grp_w = 2 * scan_status_words(n);
     const size_t plan_w = 2 * scan_status_words(2 * n);
-    const size_t err_w = 64;
-    s->zero_words = hist_w + sort_w + grp_w + plan_w + err_w;
-    s->zero_words = (s->zero_words + 3) & ~size_t(3);
-    s->zero.alloc(s->zero_words + 8);
+    // The per-step cleared words end with the readback's error words; the
+    // rest of the readback block follows them, outside the cleared range.
+    const size_t rb_w = sizeof(Readback) / sizeof(uint32_t);
+    size_t off = hist_w + sort_w;
+    off = (off + 1) & ~size_t(1); // 64-bit words must be 8-byte aligned
+    const size_t group_off = off;
+    off += grp_w;
+    const size_t plan_off = off;
+    off += plan_w;
+    s->zero.alloc(off + rb_w + 2);
     uint32_t* z = s->zero.p;
     s->hist = z;
     s->sort_status = z + hist_w;
-    // 64-bit words must be 8-byte aligned
-    size_t off = hist_w + sort_w;
-    off = (off + 1) & ~size_t(1);
-    s->group_status = reinterpret_cast<unsigned long long*>(z + off);
-    off += grp_w;
-    s->plan_status = reinterpret_cast<unsigned long long*>(z + off);
-    off += plan_w;
-    s->errors = reinterpret_cast<int*>(z + off);
-    s->zero_words = off + err_w;
-    s->box.alloc(5);
+    s->group_status = reinterpret_cast<unsigned long long*>(z + group_off);
+    s->plan_status = reinterpret_cast<unsigned long long*>(z + plan_off);
+    s->mid = reinterpret_cast<Readback*>(z + off);
+    s->errors = s->mid->errors;
+    s->box = s->mid->box;
+    s->zero_words = off + kReadbackClearedWords;
     s->partials.alloc(bbox_partial_doubles());
     s->scalars.alloc(16);
     s->counters.alloc(4);
@@ -110,16 +113,17 @@ void ensure_octree(nbh_sim* s, size_t m)
     const size_t cap = m + m / 4 + 1024;
     s->children.alloc(8 * cap);
     for (auto* b : {&s->oparent, &s->level, &s->ofirst, &s->ocount, &s->pending,
-                    &s->child_count, &s->size, &s->local, &s->pre, &s->order})
+                    &s->child_count, &s->order})
         b->alloc(cap);
-    s->level_info.alloc(level_info_ints());
     s->sum4.alloc(cap);
+    s->aux.alloc(cap);
+    s->crit.alloc(cap);
+    s->link.alloc(cap);
+    s->pos.alloc(cap);
     s->prefix.alloc(cap * s->key_bytes);
     for (auto* b : {&s->mass, &s->comx, &s->comy, &s->comz, &s->cenx, &s->ceny,
-                    &s->cenz, &s->half, &s->crit})
+                    &s->cenz, &s->half})
         b->alloc(cap);
-    s->hcount.alloc(cap);
-    s->hoffset.alloc(cap);
     if (mixed(s)) {
         s->w32node.alloc(2 * cap); // interleaved {node, link} pairs
         if (s->cfg.theta >= kContainsTheta)
@@ -152,45 +156,41 @@ void rebuild(nbh_sim* s, bool kick_drift)
         auto& c = s->acc[s->cur];
         launch_kick_drift_bbox(st, s->x.p, s->y.p, s->z.p, s->vx.p, s->vy.p,
                                s->vz.p, c[0].p, c[1].p, c[2].p, n,
-                               s->cfg.time_step, s->box.p, s->partials.p,
+                               s->cfg.time_step, s->box, s->partials.p,
                                s->errors + ERR_BBOX, s->m.p, s->pos4.p);
     } else {
         nbh_context tmp;
         tmp.stream = st;
-        launch_bbox(&tmp, s->x.p, s->y.p, s->z.p, n, s->box.p, s->partials.p,
+        launch_bbox(&tmp, s->x.p, s->y.p, s->z.p, n, s->box, s->partials.p,
                     s->errors + ERR_BBOX, s->m.p, s->pos4.p);
     }
     s->mark(PH_ENCODE);
     K* keys = reinterpret_cast<K*>(s->keys.p);
-    launch_encode<BITS>(st, s->x.p, s->y.p, s->z.p, n, s->box.p, keys,
+    launch_encode<BITS>(st, s->x.p, s->y.p, s->z.p, n, s->box, keys,
                         s->idx.p, s->hist);
     s->mark(PH_SORT);
     launch_radix_passes<K>(st, keys, s->idx.p, reinterpret_cast<K*>(s->keys_alt.p),
                            s->idx_alt.p, n, s->passes, s->hist, s->sort_status);
     s->mark(PH_GATHER);
     launch_gather(st, s->idx.p, s->m.p, s->x.p, s->y.p, s->z.p, n, s->rec64.p,
-                  mixed(s) ? s->rec32.p : nullptr, s->box.p, s->pos4.p);
+                  mixed(s) ? s->rec32.p : nullptr, s->box, s->pos4.p);
     s->mark(PH_GROUPS);
-    int* d_ng = s->scalars.p + 1;
-    int* d_nm = s->scalars.p + 2;
+    int* d_ng = &s->mid->ngroups;
+    int* d_nm = &s->mid->nnodes;
     K* unique = reinterpret_cast<K*>(s->unique.p);
     launch_leaf_groups<K>(st, keys, ni, unique, s->gfirst.p, s->gcount.p, d_ng,
                           s->group_status);
     s->mark(PH_KARRAS);
     launch_karras<K>(st, unique, d_ng, ni, s->left.p, s->right.p, s->tparent.p,
-                     s->rfirst.p, s->rlast.p, s->plen.p);
+                     s->rfirst.p, s->rlast.p, s->plen.p, s->pplen.p);
     s->mark(PH_PLAN);
     launch_plan_scan<BITS>(st, d_ng, ni, s->tparent.p, s->plen.p, unique,
                            s->plevel.p, s->pcount.p, s->poffset.p, d_nm,
-                           s->errors + ERR_PLAN, s->plan_status);
-    // The one mid-step readback: group and node counts, stage errors, box.
-    NBH_HIP(hipMemcpyAsync(&s->rb->ngroups, d_ng, 2 * sizeof(int),
-                           hipMemcpyDeviceToHost, st));
-    NBH_HIP(hipMemcpyAsync(s->rb->errors, s->errors,
-                           ERR_WORD_COUNT * sizeof(int), hipMemcpyDeviceToHost,
-                           st));
-    NBH_HIP(hipMemcpyAsync(s->rb->box, s->box.p, 5 * sizeof(double),
-                           hipMemcpyDeviceToHost, st));
+                           s->errors + ERR_PLAN, s->plan_status, s->mid->level_count,
+                           s->pplen.p);
+    // The one mid-step readback: stage errors, group and node counts, the
+    // internal nodes per level and the box, in one copy.
+    NBH_HIP(hipMemcpyAsync(s->rb, s->mid, sizeof(Readback), hipMemcpyDeviceToHost, st));
     NBH_HIP(hipStreamSynchronize(st));
     if (s->rb->errors[ERR_BBOX] != 0)
         fail(NBH_DOMAIN_ERROR,
@@ -209,22 +209,25 @@ void rebuild(nbh_sim* s, bool kick_drift)
         fail(NBH_OVERFLOW_ERROR,
              "The computed octree node count does not fit in int");
     s->nnodes = s->rb->nnodes;
+    s->starts = level_starts(s->rb->level_count);
     ensure_octree(s, static_cast<size_t>(s->nnodes));
     s->mark(PH_OCTREE);
     nbh_octree_view o = s->octree_view();
+    const LevelArrays la = s->level_arrays();
     launch_octree_build<BITS>(st, o, s->ngroups, s->tparent.p, s->rfirst.p,
                               unique, s->gfirst.p, s->gcount.p, s->plevel.p,
                               s->pcount.p, s->poffset.p, s->errors + ERR_OCTREE,
-                              false);
+                              false, s->rec64.p, la.sum4, la.aux);
+    launch_level_scatter(st, d_ng, s->ngroups, s->plevel.p, s->pcount.p, s->poffset.p,
+                         s->mid->level_count, s->mid->level_cursor, s->order.p);
     s->mark(PH_COM);
     const bool contains = s->cfg.theta >= kContainsTheta;
     launch_level_passes<BITS>(
-        st, o, Layout<BITS>::bits, mixed(s), s->rec64.p, s->rec32.p,
-        s->cfg.theta, s->box.p, contains,
-        s->cfg.softening, s->size.p, s->local.p, s->crit.p, s->pre.p, s->order.p, s->level_info.p, s->sum4.p, s->poffset.p, s->pcount.p, s->ngroups,
-        s->cfg.gravitational_constant, s->w64.p, s->w64center.p, s->w32node.p, nullptr,
+        st, o, s->starts, mixed(s), s->rec64.p, s->rec32.p, s->cfg.theta, s->box, contains,
+        s->cfg.softening, la, s->order.p, s->poffset.p, s->pcount.p, s->ngroups,
+        s->cfg.gravitational_constant, s->w64.p, s->w64center.p, s->w32node.p,
         s->w32center.p, s->scalars.p + 3, s->timing ? s->ev[PH_WALK] : nullptr,
-        s->hcount.p, s->hoffset.p, s->horder.p, s->htab.p);
+        s->horder.p, s->htab.p);
 }
 
 void forces(nbh_sim* s, int buffer, unsigned long long* counters)
@@ -285,9 +288,9 @@ void sim_fill_views(nbh_sim* s)
         return;
     nbh_octree_view o = s->octree_view();
     if (s->cfg.bits_per_axis == 10)
-        launch_octree_views<10>(s->stream, o, s->sum4.p, s->box.p);
+        launch_octree_views<10>(s->stream, o, s->level_arrays(), s->box);
     else
-        launch_octree_views<21>(s->stream, o, s->sum4.p, s->box.p);
+        launch_octree_views<21>(s->stream, o, s->level_arrays(), s->box);
     s->views_fresh = true;
 }
 
@@ -567,7 +570,7 @@ int nbh_sim_structure_view(nbh_sim* s, nbh_sim_structure* out)
         out->edge_node_count = s->pcount.p;
         out->edge_node_offset = s->poffset.p;
         out->target_order = s->horder.p;
-        out->box = s->box.p;
+        out->box = s->box;
     });
 }
 

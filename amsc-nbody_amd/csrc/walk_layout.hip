@@ -31,6 +31,7 @@
 // each separation is relative to the separation itself -- and sums each
 // target's acceleration with Kahan compensation.
 #include "octree_device.hpp"
+#include "sim_internal.hpp"
 #include "walk_common.hpp"
 #include "hilbert.hpp"
 
@@ -123,20 +124,11 @@ __device__ __forceinline__ int preorder(const nbh_octree_view& o,
 }
 
 // --------------------------------------------------------- walk records
-// Outputs of the walk build (records indexed by pre-order position).
-// The mixed traversal's target order along the Hilbert curve (see
-// hilbert.cpp), built alongside the level passes; all pointers are null
-// when the order is not wanted.  A node's curve offset is the rank of its
-// subtree's first particle along the curve: the bottom-up pass writes each
-// child's offset relative to its parent, the top-down pass adds the
+// The traversals' target order along the Hilbert curve (see hilbert.cpp) is
+// built alongside the level passes.  A node's curve offset is the rank of
+// its subtree's first particle along the curve: the bottom-up pass writes
+// each child's offset relative to its parent, the top-down pass adds the
 // parent's absolute offset (as for pre-order positions).
-struct HilbertOut {
-    int* count;          // subtree particle count per node (bottom-up)
-    int* offset;         // curve offset, relative to the parent until made absolute
-    uint32_t* order;     // curve rank -> sorted particle index
-    const uint8_t* tab;  // HilbertTables: rank[24][8] then next[24][8]
-};
-
 constexpr int kHilbertTabBytes = 2 * 8 * kHilbertStates;
 
 // Curve orientation state of the level-L cell with (left-aligned) Morton
@@ -153,6 +145,7 @@ __device__ __forceinline__ int curve_state(typename Layout<BITS>::Key p, int L,
     return st;
 }
 
+// Outputs of the walk build (records indexed by pre-order position).
 struct WalkOut {
     Walk64Rec* w64;
     double4* w64center;
@@ -162,79 +155,75 @@ struct WalkOut {
     double G; // FP64 records hold G m
 };
 
-// The record of node v at pre-order position `pre`.  GENERIC: a
-// caller-supplied octree, with the reference's per-node failure conditions
-// encoded as error kinds and the opening radius computed here.  Otherwise:
-// the pipeline's own octree with its precomputed opening radius, packed
-// {com, mass} in sum4[v] and the cell geometry {centre, half} in `geom`.
-template <bool GENERIC, bool MIXED>
-__device__ __forceinline__ void write_record(
-    const nbh_octree_view& o, int v, int pre, const int* __restrict__ size,
-    const double* __restrict__ crit_in, double theta,
-    const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, bool need_center, double eps,
-    const WalkOut& w, const double4* __restrict__ sum4, double4 geom)
-{
-    double4* w64center = w.w64center;
-    int4* w32node = w.w32node;
-    int4* w32link = w.w32link;
-    int4* w32center = w.w32center;
-    const int skip = pre + size[v];
-    const int pc = o.particle_count[v];
-    const double h = GENERIC ? o.half_size[v] : geom.w;
+// What one node's walk record states: packed {com, mass}, cell geometry
+// {centre, half}, the opening radius (internal and childless nodes), the
+// kind, a leaf's particle range and the position after the subtree.
+struct NodeVals {
     double4 mc;
-    if (GENERIC)
-        mc = make_double4(o.com_x[v], o.com_y[v], o.com_z[v], o.mass[v]);
-    else
-        mc = sum4[v];
-    int kind;
-    int first = 0, count = 0;
-    if (GENERIC) {
-        const int nch = o.child_count[v];
-        if (!(h > 0.0)) {
-            kind = KIND_ERROR + 8;
-        } else if (pc > 0) {
-            first = o.first_particle[v];
-            count = pc;
-            if (nch != 0)
-                kind = KIND_ERROR + 7;
-            else if (first < 0 || pc > o.n_particles ||
-                     first > o.n_particles - pc)
-                kind = KIND_ERROR + 2;
-            else
-                kind = KIND_LEAF;
-        } else if (o.mass[v] == 0.0) {
-            kind = KIND_SKIP;
-        } else if (nch == 0) {
-            kind = KIND_CHILDLESS;
-        } else {
-            kind = KIND_INTERNAL;
-        }
+    double4 geom;
+    double crit;
+    int kind, first, count, skip;
+};
+
+// The reference's per-node conditions for a caller-supplied octree
+// (barnes_hut.cu:101-162, 273-294), encoded as record kinds, with the
+// opening radius computed here.
+__device__ __forceinline__ NodeVals generic_vals(const nbh_octree_view& o, int v,
+                                                 int pre, const int* __restrict__ size,
+                                                 double theta)
+{
+    NodeVals n;
+    n.skip = pre + size[v];
+    n.mc = make_double4(o.com_x[v], o.com_y[v], o.com_z[v], o.mass[v]);
+    n.geom = make_double4(o.center_x[v], o.center_y[v], o.center_z[v], o.half_size[v]);
+    n.first = 0;
+    n.count = 0;
+    n.crit = 0.0;
+    const int pc = o.particle_count[v];
+    const int nch = o.child_count[v];
+    const double h = n.geom.w;
+    if (!(h > 0.0)) {
+        n.kind = KIND_ERROR + 8;
+    } else if (pc > 0) {
+        n.first = o.first_particle[v];
+        n.count = pc;
+        if (nch != 0)
+            n.kind = KIND_ERROR + 7;
+        else if (n.first < 0 || pc > o.n_particles || n.first > o.n_particles - pc)
+            n.kind = KIND_ERROR + 2;
+        else
+            n.kind = KIND_LEAF;
+    } else if (o.mass[v] == 0.0) {
+        n.kind = KIND_SKIP;
+    } else if (nch == 0) {
+        n.kind = KIND_CHILDLESS;
     } else {
-        if (pc > 0) {
-            kind = KIND_LEAF;
-            first = o.first_particle[v];
-            count = pc;
-        } else if (mc.w == 0.0) {
-            kind = KIND_SKIP;
-        } else {
-            kind = KIND_INTERNAL;
-        }
+        n.kind = KIND_INTERNAL;
     }
-    const double m = mc.w;
-    const double cx = mc.x, cy = mc.y, cz = mc.z;
-    double crit = 0.0;
-    if (kind == KIND_INTERNAL || kind == KIND_CHILDLESS) {
-        if (GENERIC) {
-            const double ox = cx - o.center_x[v];
-            const double oy = cy - o.center_y[v];
-            const double oz = cz - o.center_z[v];
-            const double o2 = fma(oz, oz, fma(ox, ox, oy * oy));
-            crit = (2.0 * h) / theta + sqrt(o2);
-        } else {
-            crit = crit_in[v];
-        }
+    if (n.kind == KIND_INTERNAL || n.kind == KIND_CHILDLESS) {
+        const double ox = n.mc.x - n.geom.x;
+        const double oy = n.mc.y - n.geom.y;
+        const double oz = n.mc.z - n.geom.z;
+        const double o2 = fma(oz, oz, fma(ox, ox, oy * oy));
+        n.crit = (2.0 * h) / theta + sqrt(o2);
     }
+    return n;
+}
+
+// The record of a node at pre-order position `pre`.
+template <bool MIXED>
+__device__ __forceinline__ void emit_record(const NodeVals& nv, int pre,
+                                            const double4* __restrict__ rec64,
+                                            const int4* __restrict__ rec32,
+                                            const double* __restrict__ box,
+                                            bool need_center, double eps,
+                                            const WalkOut& w)
+{
+    const int kind = nv.kind;
+    const int first = nv.first;
+    int count = nv.count;
+    const double4 mc = nv.mc;
+    const double crit = nv.crit;
     if (!MIXED) {
         // One-particle leaves and zero-mass nodes are always-taken
         // monopoles (thr = -inf): a leaf's exact particle term (the walk
@@ -249,12 +238,12 @@ __device__ __forceinline__ void write_record(
             r.z = p.z;
             r.m = p.w;
         } else {
-            r.x = cx;
-            r.y = cy;
-            r.z = cz;
-            r.m = m;
+            r.x = mc.x;
+            r.y = mc.y;
+            r.z = mc.z;
+            r.m = mc.w;
         }
-        r.skip = skip;
+        r.skip = nv.skip;
         r.first = (kind == KIND_LEAF) ? first : -1;
         r.count = (kind == KIND_LEAF) ? count : 0;
         r.gm = w.G * r.m;
@@ -270,11 +259,10 @@ __device__ __forceinline__ void write_record(
         }
         w.w64[pre] = r;
         if (need_center) {
-            double4 c = GENERIC ? make_double4(o.center_x[v], o.center_y[v], o.center_z[v], h)
-                                : geom;
+            double4 c = nv.geom;
             if (fast)
                 c.w = -1.0;
-            w64center[pre] = c;
+            w.w64center[pre] = c;
         }
     } else {
         const double side = box[3];
@@ -284,13 +272,13 @@ __device__ __forceinline__ void write_record(
         const double oz = box[2] - side / 2.0;
         const double lim = 2147483647.0;
         if (kind == KIND_LEAF && count == 1) {
-            w32node[2 * static_cast<size_t>(pre)] = rec32[first];
+            w.w32node[2 * static_cast<size_t>(pre)] = rec32[first];
         } else {
-            const int qx = static_cast<int>(fmin(fmax(rint((cx - ox) * scale), 0.0), lim));
-            const int qy = static_cast<int>(fmin(fmax(rint((cy - oy) * scale), 0.0), lim));
-            const int qz = static_cast<int>(fmin(fmax(rint((cz - oz) * scale), 0.0), lim));
-            const float gm = static_cast<float>(m / box[4]);
-            w32node[2 * static_cast<size_t>(pre)] = make_int4(qx, qy, qz, __float_as_int(gm));
+            const int qx = static_cast<int>(fmin(fmax(rint((mc.x - ox) * scale), 0.0), lim));
+            const int qy = static_cast<int>(fmin(fmax(rint((mc.y - oy) * scale), 0.0), lim));
+            const int qz = static_cast<int>(fmin(fmax(rint((mc.z - oz) * scale), 0.0), lim));
+            const float gm = static_cast<float>(mc.w / box[4]);
+            w.w32node[2 * static_cast<size_t>(pre)] = make_int4(qx, qy, qz, __float_as_int(gm));
         }
         // The traversal compares the softened squared distance d^2 + eps^2
         // (its interaction's r^2) with crit^2 + eps^2, both in fixed-point
@@ -317,27 +305,21 @@ __device__ __forceinline__ void write_record(
             crit2 = -1.0f;
         else if (kind_out != KIND_INTERNAL)
             crit2 = __int_as_float(0x7F800000);
-        w32link[2 * static_cast<size_t>(pre)] = make_int4(__float_as_int(crit2), skip,
-                                 kind_out | (count << 8), first);
+        w.w32link[2 * static_cast<size_t>(pre)] =
+            make_int4(__float_as_int(crit2), nv.skip, kind_out | (count << 8), first);
         if (need_center) {
-            const double gx = GENERIC ? o.center_x[v] : geom.x;
-            const double gy = GENERIC ? o.center_y[v] : geom.y;
-            const double gz = GENERIC ? o.center_z[v] : geom.z;
-            const int qx = static_cast<int>(rint((gx - ox) * scale));
-            const int qy = static_cast<int>(rint((gy - oy) * scale));
-            const int qz = static_cast<int>(rint((gz - oz) * scale));
-            const int qh = fast ? -1 : static_cast<int>(fmin(rint(h * scale), lim));
-            w32center[pre] = make_int4(qx, qy, qz, qh);
+            const int qx = static_cast<int>(rint((nv.geom.x - ox) * scale));
+            const int qy = static_cast<int>(rint((nv.geom.y - oy) * scale));
+            const int qz = static_cast<int>(rint((nv.geom.z - oz) * scale));
+            const int qh = fast ? -1 : static_cast<int>(fmin(rint(nv.geom.w * scale), lim));
+            w.w32center[pre] = make_int4(qx, qy, qz, qh);
         }
     }
 }
 
-template <bool GENERIC, bool MIXED>
 __global__ __launch_bounds__(256) void walk_build_kernel(
     nbh_octree_view o, const int* __restrict__ size,
-    const int* __restrict__ local, const double* __restrict__ crit_in,
-    double theta, const double4* __restrict__ rec64,
-    const int4* __restrict__ rec32, const double* __restrict__ box,
+    const int* __restrict__ local, double theta, const double4* __restrict__ rec64,
     bool need_center, double eps, WalkOut w, int* __restrict__ wcount)
 {
     const int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -348,115 +330,95 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
         return;
     if (v == 0)
         *wcount = size[0] < o.n_nodes ? size[0] : o.n_nodes;
-    write_record<GENERIC, MIXED>(o, v, pre, size, crit_in, theta, rec64, rec32,
-                                 box, need_center, eps, w, nullptr,
-                                 make_double4(0.0, 0.0, 0.0, 0.0));
+    emit_record<false>(generic_vals(o, v, pre, size, theta), pre, rec64, nullptr, nullptr,
+                       need_center, eps, w);
 }
 
+
 // ------------------------------------------ level-synchronous passes
-// Bottom-up over one octree level: every child of a level-L node sits at
-// level L+1 and was completed by the previous launch, so no arrival
-// counters (and no cache write-back/invalidate per node) are needed.  A
-// node's values are the reference's: leaf mass/COM summed over its
-// particles in sorted order (octree_physics.cu:94-153), internal nodes
-// combined in octant order 0..7 (octree_physics.cu:211-243), cell geometry
-// from the prefix path (octree_geometry.cu:56-102), the opening radius
-// side/theta + |com - centre| (barnes_hut.cu:277-291); plus the subtree size
-// and the pre-order offsets of the children (octants 7 -> 0).
+// Bottom-up over octree levels: every child of a level-L node sits at level
+// L+1 and was completed by the previous launch, so no arrival counters (and
+// no cache write-back/invalidate per node) are needed.  A node's values are
+// the reference's: leaf mass/COM summed over its particles in sorted order
+// (octree_physics.cu:94-153, done by the octree materialisation), internal
+// nodes combined in octant order 0..7 (octree_physics.cu:211-243), cell
+// geometry from the prefix path (octree_geometry.cu:56-102), the opening
+// radius side/theta + |com - centre| (barnes_hut.cu:277-291); plus the
+// subtree size, the pre-order offsets of the children (octants 7 -> 0) and
+// the children's Hilbert-curve offsets.
 template <int BITS>
 __global__ __launch_bounds__(256) void level_up_kernel(
-    nbh_octree_view o, int L, const double4* __restrict__ rec, double theta,
-    const double* __restrict__ box, int* __restrict__ size,
-    int* __restrict__ local, double* __restrict__ crit, bool collapse,
-    const int* __restrict__ order, const int* __restrict__ start,
-    double4* __restrict__ sum4, HilbertOut h)
+    nbh_octree_view o, int L, int begin, int end, const int* __restrict__ order,
+    LevelArrays la, const double* __restrict__ box, double theta, bool collapse,
+    const uint8_t* __restrict__ tab)
 {
     using Key = typename Layout<BITS>::Key;
     __shared__ uint8_t s_tab[kHilbertTabBytes];
-    if (h.count)
-        for (int t = static_cast<int>(threadIdx.x); t < kHilbertTabBytes; t += blockDim.x)
-            s_tab[t] = h.tab[t];
+    for (int t = static_cast<int>(threadIdx.x); t < kHilbertTabBytes; t += blockDim.x)
+        s_tab[t] = tab[t];
     __syncthreads();
     const double rcx = box[0], rcy = box[1], rcz = box[2], side = box[3];
-    const Key* prefix = static_cast<const Key*>(o.prefix);
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
-    const int end = start[L + 1];
-    for (int k = start[L] + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-         k < end; k += stride) {
+    for (int k = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); k < end;
+         k += stride) {
         const int v = order[k];
-        const Key pv = prefix[v];
-        double gx, gy, gz, gh;
-        int code = 0;
-        node_geometry<BITS>(pv, L, rcx, rcy, rcz, side, gx, gy, gz, gh, code);
+        if (v < 0 || v >= o.n_nodes)
+            continue;
+        const Key pv = static_cast<const Key*>(o.prefix)[v];
+        const int4 c0 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v)];
+        const int4 c1 =
+            reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v) + 1];
+        const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        // every present child's words are requested before any is used
+        double4 cs[8];
+        int4 ca[8]; // size, hcount, pc
+#pragma unroll
+        for (int oc = 0; oc < 8; ++oc) {
+            cs[oc] = make_double4(0.0, 0.0, 0.0, 0.0);
+            ca[oc] = make_int4(0, 0, 0, 0);
+            if (ch[oc] != -1) {
+                cs[oc] = la.sum4[ch[oc]];
+                ca[oc] = la.aux[ch[oc]];
+            }
+        }
         double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
-        const int pc = o.particle_count[v];
-        int nch = 0, only = 0;
-        if (pc > 0) {
-            const int f = o.first_particle[v];
-            for (int k = 0; k < pc; ++k) {
-                const double4 p = rec[f + k];
-                tm += p.w;
-                wx = fma(p.w, p.x, wx);
-                wy = fma(p.w, p.y, wy);
-                wz = fma(p.w, p.z, wz);
-            }
-            size[v] = 1;
-            if (h.count)
-                h.count[v] = pc;
-        } else {
-            const int4 c0 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v)];
-            const int4 c1 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v) + 1];
-            const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        int nch = 0, only_pc = 0; // child count, the last child's particle count
 #pragma unroll
-            for (int oc = 0; oc < 8; ++oc) {
-                if (ch[oc] == -1)
-                    continue;
-                const double4 c = sum4[ch[oc]];
-                tm += c.w;
-                wx = fma(c.w, c.x, wx);
-                wy = fma(c.w, c.y, wy);
-                wz = fma(c.w, c.z, wz);
-                only = ch[oc];
-                ++nch;
-            }
-            // A collapsed single-child node takes no walk position of its
-            // own: its child starts at the node's position and overwrites
-            // the node's record on the next top-down level.  Only internal
-            // children and single-particle leaves qualify (a leaf holding
-            // several particles is summed exactly, never as a monopole).
-            // Curve offsets of the children: in v's orientation state they
-            // take consecutive particle ranges in the order of their ranks.
-            if (h.count) {
-                int cnt[8];
-                int hc = 0;
+        for (int oc = 0; oc < 8; ++oc) {
+            if (ch[oc] == -1)
+                continue;
+            const double4 c = cs[oc];
+            tm += c.w;
+            wx = fma(c.w, c.x, wx);
+            wy = fma(c.w, c.y, wy);
+            wz = fma(c.w, c.z, wz);
+            only_pc = ca[oc].z;
+            ++nch;
+        }
+        // Curve offsets of the children: in v's orientation state they take
+        // consecutive particle ranges in the order of their ranks.
+        int hc = 0;
 #pragma unroll
-                for (int oc = 0; oc < 8; ++oc) {
-                    cnt[oc] = ch[oc] != -1 ? h.count[ch[oc]] : 0;
-                    hc += cnt[oc];
-                }
-                h.count[v] = hc;
-                const uint8_t* rank =
-                    s_tab + 8 * curve_state<BITS>(pv, L, s_tab + 8 * kHilbertStates);
+        for (int oc = 0; oc < 8; ++oc)
+            hc += ca[oc].y;
+        const uint8_t* rank =
+            s_tab + 8 * curve_state<BITS>(pv, L, s_tab + 8 * kHilbertStates);
+        // A collapsed single-child node takes no walk position of its own:
+        // its child starts at the node's position and overwrites the node's
+        // record on the next top-down level.  Only internal children and
+        // single-particle leaves qualify (a leaf holding several particles is
+        // summed exactly, never as a monopole).
+        int off = (collapse && nch == 1 && only_pc <= 1) ? 0 : 1;
 #pragma unroll
-                for (int oc = 0; oc < 8; ++oc) {
-                    if (ch[oc] == -1)
-                        continue;
-                    int before = 0;
+        for (int oc = 7; oc >= 0; --oc) {
+            if (ch[oc] == -1)
+                continue;
+            int before = 0;
 #pragma unroll
-                    for (int b = 0; b < 8; ++b)
-                        before += (rank[b] < rank[oc]) ? cnt[b] : 0;
-                    h.offset[ch[oc]] = before;
-                }
-            }
-            int off = (collapse && nch == 1 && o.particle_count[only] <= 1) ? 0 : 1;
-#pragma unroll
-            for (int oc = 7; oc >= 0; --oc) {
-                if (ch[oc] == -1)
-                    continue;
-                local[ch[oc]] = off;
-                off += size[ch[oc]];
-            }
-            size[v] = off;
+            for (int b = 0; b < 8; ++b)
+                before += (rank[b] < rank[oc]) ? ca[b].y : 0;
+            la.link[ch[oc]] = make_int2(off, before);
+            off += ca[oc].x;
         }
         double cx = 0.0, cy = 0.0, cz = 0.0;
         if (tm > 0.0) {
@@ -465,182 +427,160 @@ __global__ __launch_bounds__(256) void level_up_kernel(
             cy = wy * inv;
             cz = wz * inv;
         }
-        sum4[v] = make_double4(cx, cy, cz, tm);
-        if (pc <= 0) {
-            const double ox = cx - gx;
-            const double oy = cy - gy;
-            const double oz = cz - gz;
-            const double o2 = fma(oz, oz, fma(ox, ox, oy * oy));
-            crit[v] = (2.0 * gh) / theta + sqrt(o2);
-        }
-        if (v == 0)
-            local[0] = 0;
+        double gx, gy, gz, gh;
+        int code = 0;
+        node_geometry<BITS>(pv, L, rcx, rcy, rcz, side, gx, gy, gz, gh, code);
+        const double ox = cx - gx;
+        const double oy = cy - gy;
+        const double oz = cz - gz;
+        const double o2 = fma(oz, oz, fma(ox, ox, oy * oy));
+        la.sum4[v] = make_double4(cx, cy, cz, tm);
+        la.aux[v] = make_int4(off, hc, 0, 0);
+        la.crit[v] = (2.0 * gh) / theta + sqrt(o2);
     }
 }
 
-// Top-down over one level: pre-order position = parent's + own offset, then
-// the node's walk record.
+// Top-down over one level: pre-order position = parent's + own offset, the
+// curve offset made absolute, then the node's walk record.  The root
+// (v = 0, level 0) also writes the walk length and the terminal record at
+// it ("error 4, skip to the end"): a walk reaches that position only by
+// finishing, and the traversal's internal-node loop stops at any record of
+// another kind, so it needs no separate bound test.  The record buffers
+// hold more than n_nodes records, and the position may still hold a record
+// of an earlier, longer walk.
 template <int BITS, bool MIXED>
 __global__ __launch_bounds__(256) void level_down_kernel(
-    nbh_octree_view o, int L, const int* __restrict__ size,
-    const int* __restrict__ local, int* __restrict__ pre_of,
-    const double* __restrict__ crit, double theta,
-    const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, bool need_center, double eps,
-    WalkOut w, int* __restrict__ wcount, const int* __restrict__ order,
-    const int* __restrict__ start, const double4* __restrict__ sum4,
-    HilbertOut h)
+    nbh_octree_view o, int L, int begin, int end, const int* __restrict__ order,
+    LevelArrays la, const double4* __restrict__ rec64, const int4* __restrict__ rec32,
+    const double* __restrict__ box, bool need_center, double eps, WalkOut w,
+    int* __restrict__ wcount)
 {
     using Key = typename Layout<BITS>::Key;
-    const Key* prefix = static_cast<const Key*>(o.prefix);
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
-    const int end = start[L + 1];
-    for (int k = start[L] + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-         k < end; k += stride) {
+    for (int k = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); k < end;
+         k += stride) {
         const int v = order[k];
-        int pre = 0;
-        if (L == 0) {
-            if (h.count)
-                h.offset[v] = 0;
-            const int wc = size[0] < o.n_nodes ? size[0] : o.n_nodes;
+        if (v < 0 || v >= o.n_nodes)
+            continue;
+        const int4 a = la.aux[v];
+        int pre = 0, hoff = 0;
+        if (v == 0) {
+            const int wc = a.x < o.n_nodes ? a.x : o.n_nodes;
             *wcount = wc;
-            // Terminal record at the walk length ("error 4, skip to the
-            // end"): a walk reaches position wc only by finishing, and the
-            // traversal's internal-node loop stops at any record of another
-            // kind, so it needs no separate bound test.  The buffers hold
-            // more than n_nodes records, and position wc may still hold a
-            // record of an earlier, longer walk.
             if (MIXED)
                 w.w32link[2 * static_cast<size_t>(wc)] =
                     make_int4(0x7F800000, wc, KIND_ERROR + 4, 0);
             else
                 w.w64[wc] = error_record64(wc);
         } else {
-            const int parent = o.parent[v];
-            pre = pre_of[parent] + local[v];
-            if (h.count)
-                h.offset[v] += h.offset[parent];
+            const int2 lk = la.link[v];
+            const int2 p = la.pos[o.parent[v]];
+            pre = p.x + lk.x;
+            hoff = p.y + lk.y;
         }
-        pre_of[v] = pre;
+        la.pos[v] = make_int2(pre, hoff);
         if (pre < 0 || pre >= o.n_nodes)
             continue;
-        double4 geom = make_double4(0.0, 0.0, 0.0, 0.0);
+        NodeVals nv;
+        nv.mc = la.sum4[v];
+        nv.geom = make_double4(0.0, 0.0, 0.0, 0.0);
+        nv.crit = la.crit[v];
+        nv.kind = (nv.mc.w == 0.0) ? KIND_SKIP : KIND_INTERNAL;
+        nv.first = 0;
+        nv.count = 0;
+        nv.skip = pre + a.x;
         if (need_center) {
             int code = 0;
-            node_geometry<BITS>(prefix[v], L, box[0], box[1], box[2], box[3],
-                                geom.x, geom.y, geom.z, geom.w, code);
+            node_geometry<BITS>(static_cast<const Key*>(o.prefix)[v], L, box[0], box[1],
+                                box[2], box[3], nv.geom.x, nv.geom.y, nv.geom.z,
+                                nv.geom.w, code);
         }
-        write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32,
-                                   box, need_center, eps, w, sum4, geom);
+        emit_record<MIXED>(nv, pre, rec64, rec32, box, need_center, eps, w);
     }
 }
 
 // ------------------------------------------------- nodes grouped by level
-// The octree numbers nodes by radix-tree node, not by level.  A counting
-// sort groups them by level once per build, so that each level pass reads
-// only its own nodes.  Order within a level is irrelevant: every node of a
-// level is processed independently.  Three launches with the same grid:
-// per-block level counts, one scan over (level, block), and a scatter
-// through per-block cursors in LDS (no global atomics).
-// info layout: [0, kMaxLevels + 1) level starts (start[L + 1] ends level L),
-// then the kMaxLevels x blocks per-block counts / bases, level-major.
-constexpr int kMaxLevels = 32; // 63-bit keys give at most 22 levels
-constexpr int kLevelBlocks = 256;
-constexpr int kLevelCounts = kMaxLevels * kLevelBlocks;
-constexpr int kLevelInfoInts = kMaxLevels + 1 + kLevelCounts;
+// The octree numbers nodes by radix-tree node, not by level.  The radix ->
+// octree plan already fixes every octree node's level: internal radix node
+// r owns the chain of octree nodes 1 + offset[r] + k at levels
+// level[r] - count[r] + 1 + k, k < count[r] (octree_builder.cu:153-174), a
+// radix leaf owns one octree leaf, and the root is node 0 at level 0.  The
+// plan scan counts the internal nodes per level (the counts join the
+// mid-step readback and size the level launches); this kernel groups the
+// nodes by level.  A block counts its nodes per level in LDS, reserves
+// ranges of each level with one global add per level, and hands out slots
+// in LDS.  Order within a level is irrelevant: every node of a level is
+// processed independently.
+constexpr int kScatterItems = 4;
+constexpr int kScatterThreads = 256;
 
-// Internal nodes only: leaves are handled by the per-group leaf passes.
-__device__ __forceinline__ int node_level(const nbh_octree_view& o, int v)
+__global__ __launch_bounds__(kScatterThreads) void level_scatter_kernel(
+    const int* __restrict__ d_nleaves, const int* __restrict__ plevel,
+    const int* __restrict__ pcount, const int* __restrict__ poffset,
+    const int* __restrict__ level_count, int* __restrict__ cursor,
+    int* __restrict__ order)
 {
-    if (o.particle_count[v] > 0)
-        return -1;
-    const int L = o.level[v];
-    return (L >= 0 && L < kMaxLevels) ? L : -1;
-}
-
-__global__ __launch_bounds__(256) void level_count_kernel(nbh_octree_view o,
-                                                          int* __restrict__ info)
-{
-    __shared__ int cnt[kMaxLevels];
-    if (threadIdx.x < kMaxLevels)
-        cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const int stride = static_cast<int>(gridDim.x * blockDim.x);
-    for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-         v < o.n_nodes; v += stride) {
-        const int L = node_level(o, v);
-        if (L >= 0)
-            atomicAdd(&cnt[L], 1);
-    }
-    __syncthreads();
-    if (threadIdx.x < kMaxLevels)
-        info[kMaxLevels + 1 + threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
-}
-
-// One block of 1024 threads: exclusive scan of the level-major counts (zero
-// beyond `blocks`) staged in LDS.
-__global__ __launch_bounds__(1024) void level_scan_kernel(int* __restrict__ info,
-                                                          int blocks)
-{
-    constexpr int kPer = kLevelCounts / 1024;
-    __shared__ int buf[kLevelCounts];
-    __shared__ int part[1024];
-    int* bc = info + kMaxLevels + 1;
-    const int total = kMaxLevels * blocks;
+    __shared__ int s_cnt[kMaxLevels];
+    __shared__ int s_slot[kMaxLevels];
     const int t = static_cast<int>(threadIdx.x);
-    for (int k = t; k < kLevelCounts; k += 1024)
-        buf[k] = k < total ? bc[k] : 0;
-    __syncthreads();
-    int sum = 0;
-#pragma unroll
-    for (int j = 0; j < kPer; ++j)
-        sum += buf[t * kPer + j];
-    part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int add = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += add;
-        __syncthreads();
-    }
-    int run = part[t] - sum;
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        const int c = buf[t * kPer + j];
-        buf[t * kPer + j] = run;
-        run += c;
-    }
-    __syncthreads();
-    for (int k = t; k < total; k += 1024)
-        bc[k] = buf[k];
     if (t < kMaxLevels)
-        info[t] = buf[t * blocks];
-    if (t == 0)
-        info[kMaxLevels] = part[1023];
-}
-
-__global__ __launch_bounds__(256) void level_scatter_kernel(nbh_octree_view o,
-                                                            const int* __restrict__ info,
-                                                            int* __restrict__ order)
-{
-    __shared__ int cursor[kMaxLevels];
-    if (threadIdx.x < kMaxLevels)
-        cursor[threadIdx.x] =
-            info[kMaxLevels + 1 + threadIdx.x * gridDim.x + blockIdx.x];
+        s_cnt[t] = 0;
     __syncthreads();
-    const int stride = static_cast<int>(gridDim.x * blockDim.x);
-    for (int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-         v < o.n_nodes; v += stride) {
-        const int L = node_level(o, v);
-        if (L >= 0)
-            order[atomicAdd(&cursor[L], 1)] = v;
+    const int nl = *d_nleaves;
+    const int ni = nl > 1 ? nl - 1 : 0;
+    const int base = static_cast<int>(blockIdx.x) * (kScatterThreads * kScatterItems);
+    int c[kScatterItems], fl[kScatterItems], fn[kScatterItems];
+#pragma unroll
+    for (int j = 0; j < kScatterItems; ++j) {
+        const int i = base + j * kScatterThreads + t;
+        const bool in = i < ni;
+        const int r = nl + i;
+        c[j] = in ? pcount[r] : 0;
+        if (c[j] < 0 || c[j] > kMaxLevels)
+            c[j] = 0; // a failed plan: reported by its own error word
+        fl[j] = in ? plevel[r] - c[j] + 1 : 0;
+        fn[j] = in ? 1 + poffset[r] : 0;
     }
+    const bool root = blockIdx.x == 0 && t == 0;
+    if (t < 64)
+        (void)wave_claim(s_cnt, 0, root);
+#pragma unroll
+    for (int j = 0; j < kScatterItems; ++j)
+        for (int k = 0; __ballot(k < c[j]) != 0ull; ++k) {
+            const int L = fl[j] + k;
+            (void)wave_claim(s_cnt, L, k < c[j] && L >= 0 && L < kMaxLevels);
+        }
+    __syncthreads();
+    if (t < 64) {
+        // level starts: exclusive sum of the per-level counts (plus the root
+        // at level 0), lanes 0..kMaxLevels-1 of the first wave
+        const int n_l = t < kMaxLevels ? level_count[t * kLevelCountStride] + (t == 0) : 0;
+        const int start = wave_inclusive_sum(n_l) - n_l;
+        if (t < kMaxLevels) {
+            const int mine = s_cnt[t];
+            s_slot[t] = start + (mine != 0 ? atomicAdd(&cursor[t * kLevelCountStride], mine) : 0);
+        }
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int slot = wave_claim(s_slot, 0, root);
+        if (root)
+            order[slot] = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kScatterItems; ++j)
+        for (int k = 0; __ballot(k < c[j]) != 0ull; ++k) {
+            const int L = fl[j] + k;
+            const bool act = k < c[j] && L >= 0 && L < kMaxLevels;
+            const int slot = wave_claim(s_slot, L, act);
+            if (act)
+                order[slot] = fn[j] + k;
+        }
 }
 
 template <int BITS>
 __global__ __launch_bounds__(256) void octree_views_kernel(
-    nbh_octree_view o, const double4* __restrict__ sum4,
-    const double* __restrict__ box)
+    nbh_octree_view o, LevelArrays la, const double* __restrict__ box)
 {
     using Key = typename Layout<BITS>::Key;
     const int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -655,7 +595,7 @@ __global__ __launch_bounds__(256) void octree_views_kernel(
     o.center_y[v] = gy;
     o.center_z[v] = gz;
     o.half_size[v] = gh;
-    const double4 mc = sum4[v];
+    const double4 mc = la.sum4[v];
     o.mass[v] = mc.w;
     o.com_x[v] = mc.x;
     o.com_y[v] = mc.y;
@@ -674,6 +614,7 @@ __global__ __launch_bounds__(256) void octree_views_kernel(
 // leaf g, node edgeNodeOffset[g] + edgeNodeCount[g] (octree numbering of
 // octree_builder.cu:49-224).  Groups, their particles and their leaves are
 // all in Morton order, so these passes read and write nearly sequentially.
+// The leaves' sums are written by the octree materialisation.
 __device__ __forceinline__ int leaf_of_group(const int* __restrict__ poffset,
                                              const int* __restrict__ pcount,
                                              const nbh_octree_view& o, int g)
@@ -682,53 +623,15 @@ __device__ __forceinline__ int leaf_of_group(const int* __restrict__ poffset,
     return (v > 0 && v < o.n_nodes && o.particle_count[v] > 0) ? v : -1;
 }
 
-// Leaf sums (octree_physics.cu:94-153, particles in sorted order) and
-// subtree size 1, ahead of the internal levels.
-__global__ __launch_bounds__(256) void leaf_up_kernel(
-    nbh_octree_view o, const int* __restrict__ poffset,
-    const int* __restrict__ pcount, int n_groups,
-    const double4* __restrict__ rec, double4* __restrict__ sum4,
-    int* __restrict__ size, HilbertOut h)
-{
-    const int g = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (g >= n_groups)
-        return;
-    const int v = leaf_of_group(poffset, pcount, o, g);
-    if (v < 0)
-        return;
-    const int f = o.first_particle[v];
-    const int pc = o.particle_count[v];
-    double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
-    for (int k = 0; k < pc; ++k) {
-        const double4 p = rec[f + k];
-        tm += p.w;
-        wx = fma(p.w, p.x, wx);
-        wy = fma(p.w, p.y, wy);
-        wz = fma(p.w, p.z, wz);
-    }
-    double cx = 0.0, cy = 0.0, cz = 0.0;
-    if (tm > 0.0) {
-        const double inv = 1.0 / tm;
-        cx = wx * inv;
-        cy = wy * inv;
-        cz = wz * inv;
-    }
-    sum4[v] = make_double4(cx, cy, cz, tm);
-    size[v] = 1;
-    if (h.count)
-        h.count[v] = pc;
-}
-
-// Leaf positions and records, after every internal level has its position.
+// Leaf positions, curve ranks and records, after every internal level has
+// its position.
 template <int BITS, bool MIXED>
 __global__ __launch_bounds__(256) void leaf_down_kernel(
     nbh_octree_view o, const int* __restrict__ poffset,
-    const int* __restrict__ pcount, int n_groups, const int* __restrict__ size,
-    const int* __restrict__ local, int* __restrict__ pre_of,
-    const double* __restrict__ crit, double theta,
+    const int* __restrict__ pcount, int n_groups, LevelArrays la,
     const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, bool need_center, double eps,
-    WalkOut w, const double4* __restrict__ sum4, HilbertOut h)
+    const double* __restrict__ box, bool need_center, double eps, WalkOut w,
+    uint32_t* __restrict__ horder)
 {
     using Key = typename Layout<BITS>::Key;
     const int g = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -738,32 +641,39 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
     if (v < 0)
         return;
     const int parent = o.parent[v];
-    const bool has_parent = parent >= 0 && parent < o.n_nodes;
-    if (h.order) {
+    if (parent < 0 || parent >= o.n_nodes)
+        return;
+    const int2 lk = la.link[v];
+    const int2 p = la.pos[parent];
+    const int f = o.first_particle[v];
+    const int pc = o.particle_count[v];
+    const int pre = p.x + lk.x;
+    const int off = p.y + lk.y;
+    la.pos[v] = make_int2(pre, off);
+    if (horder) {
         // the leaf's particles, in sorted order, at its curve offset (its
         // parent's offset is absolute once every internal level is done)
-        const int off = (v == 0) ? 0 : has_parent ? h.offset[v] + h.offset[parent] : -1;
-        const int f = o.first_particle[v];
-        const int pc = o.particle_count[v];
         if (off >= 0 && pc > 0 && off <= o.n_particles - pc)
             for (int k = 0; k < pc; ++k)
-                h.order[off + k] = static_cast<uint32_t>(f + k);
+                horder[off + k] = static_cast<uint32_t>(f + k);
     }
-    if (!has_parent)
-        return;
-    const int pre = pre_of[parent] + local[v];
-    pre_of[v] = pre;
     if (pre < 0 || pre >= o.n_nodes)
         return;
-    double4 geom = make_double4(0.0, 0.0, 0.0, 0.0);
+    NodeVals nv;
+    nv.mc = la.sum4[v];
+    nv.geom = make_double4(0.0, 0.0, 0.0, 0.0);
+    nv.crit = 0.0;
+    nv.kind = KIND_LEAF;
+    nv.first = f;
+    nv.count = pc;
+    nv.skip = pre + 1;
     if (need_center) {
         int code = 0;
-        const Key* prefix = static_cast<const Key*>(o.prefix);
-        node_geometry<BITS>(prefix[v], o.level[v], box[0], box[1], box[2], box[3],
-                            geom.x, geom.y, geom.z, geom.w, code);
+        node_geometry<BITS>(static_cast<const Key*>(o.prefix)[v], o.level[v], box[0],
+                            box[1], box[2], box[3], nv.geom.x, nv.geom.y, nv.geom.z,
+                            nv.geom.w, code);
     }
-    write_record<false, MIXED>(o, v, pre, size, crit, theta, rec64, rec32, box,
-                               need_center, eps, w, sum4, geom);
+    emit_record<MIXED>(nv, pre, rec64, rec32, box, need_center, eps, w);
 }
 
 // Every link starts as "error 4, skip to the end", so a position that the
@@ -811,60 +721,73 @@ void launch_walk_build(hipStream_t s, const nbh_octree_view& o, const int* size,
         w64, M + 1, M);
     NBH_HIP(hipGetLastError());
     const WalkOut w{w64, w64center, nullptr, nullptr, nullptr, G};
-    walk_build_kernel<true, false><<<blocks_for(M, 256), 256, 0, s>>>(
-        o, size, local, nullptr, theta, rec64, nullptr, nullptr, need_center, eps, w,
-        wcount);
+    walk_build_kernel<<<blocks_for(M, 256), 256, 0, s>>>(o, size, local, theta, rec64,
+                                                         need_center, eps, w, wcount);
     NBH_HIP(hipGetLastError());
 }
 
-// The pipeline's mass/COM + walk layout: levels max_level..0 bottom-up, then
-// 0..max_level top-down.  Each launch strides over the node list of its
-// level (`order`, grouped by a counting sort over the node levels).
-template <int BITS>
-void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
-                         bool mixed, const double4* rec64, const int4* rec32,
-                         double theta, const double* box,
-                         bool need_center, double eps, int* size, int* local, double* crit,
-                         int* pre, int* order, int* level_info, double4* sum4,
-                         const int* poffset, const int* pcount, int n_groups,
-                         double G, Walk64Rec* w64, double4* w64center, int4* w32node,
-                         int4* w32link, int4* w32center, int* wcount,
-                         hipEvent_t between, int* hcount, int* hoffset,
-                         uint32_t* horder, const uint8_t* htab)
+LevelStarts level_starts(const int* level_count)
 {
-    // Mixed records are interleaved {node, link} pairs: w32node is the array
-    // base and w32link = w32node + 1, both indexed at 2 * position.
-    (void)w32link;
-    const HilbertOut h{hcount, hoffset, horder, htab};
-    int4* w32pair_link = w32node ? w32node + 1 : nullptr;
+    LevelStarts st{};
+    long long run = 0;
+    for (int L = 0; L < kMaxLevels; ++L) {
+        st.s[L] = static_cast<int>(run);
+        const int c = level_count[L * kLevelCountStride] + (L == 0 ? 1 : 0);
+        if (c < 0)
+            fail(NBH_LOGIC_ERROR, "inconsistent octree level counts");
+        run += c;
+        if (run > 0x7FFFFFFFll)
+            fail(NBH_OVERFLOW_ERROR, "octree level counts exceed the int range");
+    }
+    st.s[kMaxLevels] = static_cast<int>(run);
+    return st;
+}
+
+void launch_level_scatter(hipStream_t s, const int* d_nleaves, int n_leaves,
+                          const int* plevel, const int* pcount, const int* poffset,
+                          const int* level_count, int* cursor, int* order)
+{
+    const size_t ni = n_leaves > 1 ? static_cast<size_t>(n_leaves) - 1 : 0;
+    const unsigned blocks = blocks_for(ni, kScatterThreads * kScatterItems);
+    level_scatter_kernel<<<blocks, kScatterThreads, 0, s>>>(d_nleaves, plevel, pcount,
+                                                            poffset, level_count, cursor,
+                                                            order);
+    NBH_HIP(hipGetLastError());
+}
+
+// The pipeline's mass/COM + walk layout: the internal levels max_level..0
+// bottom-up (the leaves' sums come from the octree materialisation), then
+// 0..max_level top-down, then the leaves' records; one launch per level,
+// sized to its node count.
+template <int BITS>
+void launch_level_passes(hipStream_t s, const nbh_octree_view& o, const LevelStarts& st,
+                         bool mixed, const double4* rec64, const int4* rec32, double theta,
+                         const double* box, bool need_center, double eps,
+                         const LevelArrays& la, const int* order, const int* poffset,
+                         const int* pcount, int n_groups, double G, Walk64Rec* w64,
+                         double4* w64center, int4* w32node, int4* w32center, int* wcount,
+                         hipEvent_t between, uint32_t* horder, const uint8_t* htab)
+{
     // Single-child nodes are collapsed into their child in the mixed walk:
     // they carry the child's mass and centre of mass, and for theta below
     // 2/sqrt(3) accepting the parent implies accepting the child, so every
     // lane's interaction list is unchanged.
     const bool collapse = mixed && theta < kContainsTheta;
-    if (max_level >= kMaxLevels)
-        fail(NBH_LOGIC_ERROR, "octree depth exceeds the level-pass limit");
-    unsigned blocks = blocks_for(o.n_nodes, 256);
-    if (blocks > 2048)
-        blocks = 2048;
-    unsigned lblocks = blocks_for(o.n_nodes, 256);
-    if (lblocks > kLevelBlocks)
-        lblocks = kLevelBlocks;
-    level_count_kernel<<<lblocks, 256, 0, s>>>(o, level_info);
-    NBH_HIP(hipGetLastError());
-    level_scan_kernel<<<1, 1024, 0, s>>>(level_info, static_cast<int>(lblocks));
-    NBH_HIP(hipGetLastError());
-    level_scatter_kernel<<<lblocks, 256, 0, s>>>(o, level_info, order);
-    NBH_HIP(hipGetLastError());
-    const int* start = level_info;
-    const unsigned gblocks = blocks_for(n_groups > 0 ? n_groups : 1, 256);
-    leaf_up_kernel<<<gblocks, 256, 0, s>>>(o, poffset, pcount, n_groups, rec64, sum4,
-                                           size, h);
-    NBH_HIP(hipGetLastError());
+    if (st.s[0] != 0 || st.s[kMaxLevels] > o.n_nodes || st.s[1] != 1)
+        fail(NBH_LOGIC_ERROR, "inconsistent octree level counts");
+    int max_level = 0;
+    for (int L = 0; L < kMaxLevels; ++L)
+        if (st.s[L + 1] > st.s[L])
+            max_level = L;
+    const auto grid = [&](int L) {
+        unsigned b = blocks_for(static_cast<size_t>(st.s[L + 1] - st.s[L]), 256);
+        return b > 8192u ? 8192u : b;
+    };
     for (int L = max_level; L >= 0; --L) {
-        level_up_kernel<BITS><<<blocks, 256, 0, s>>>(o, L, rec64, theta, box, size,
-                                                     local, crit, collapse, order,
-                                                     start, sum4, h);
+        if (st.s[L + 1] == st.s[L])
+            continue;
+        level_up_kernel<BITS><<<grid(L), 256, 0, s>>>(o, L, st.s[L], st.s[L + 1], order, la,
+                                                      box, theta, collapse, htab);
         NBH_HIP(hipGetLastError());
     }
     if (between)
@@ -874,34 +797,34 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, int max_level,
     // here on each rebuild, as is the terminal record at the walk length.
     // The error sentinel of the other positions is written once, when the
     // record buffers are allocated (launch_walk_sentinel); the buffers hold
-    // at least n_nodes + 1024 records.
-    const WalkOut w{w64, w64center, w32node, w32pair_link,
-                    w32center, G};
+    // at least n_nodes + 1024 records.  Mixed records are interleaved
+    // {node, link} pairs: w32node is the array base and the links start one
+    // int4 later, both indexed at 2 * position.
+    const WalkOut w{w64, w64center, w32node, w32node ? w32node + 1 : nullptr, w32center, G};
     for (int L = 0; L <= max_level; ++L) {
+        if (st.s[L + 1] == st.s[L])
+            continue;
         if (mixed)
-            level_down_kernel<BITS, true><<<blocks, 256, 0, s>>>(
-                o, L, size, local, pre, crit, theta, rec64, rec32, box,
-                need_center, eps, w, wcount, order, start, sum4, h);
+            level_down_kernel<BITS, true><<<grid(L), 256, 0, s>>>(
+                o, L, st.s[L], st.s[L + 1], order, la, rec64, rec32, box, need_center, eps,
+                w, wcount);
         else
-            level_down_kernel<BITS, false><<<blocks, 256, 0, s>>>(
-                o, L, size, local, pre, crit, theta, rec64, rec32, box,
-                need_center, eps, w, wcount, order, start, sum4, h);
+            level_down_kernel<BITS, false><<<grid(L), 256, 0, s>>>(
+                o, L, st.s[L], st.s[L + 1], order, la, rec64, rec32, box, need_center, eps,
+                w, wcount);
         NBH_HIP(hipGetLastError());
     }
     // Leaves last: a leaf that collapsed its single-child parent overwrites
     // the parent's record at the shared position.
+    const unsigned gblocks = blocks_for(n_groups > 0 ? n_groups : 1, 256);
     if (mixed)
         leaf_down_kernel<BITS, true><<<gblocks, 256, 0, s>>>(
-            o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
-            box, need_center, eps, w, sum4, h);
+            o, poffset, pcount, n_groups, la, rec64, rec32, box, need_center, eps, w, horder);
     else
         leaf_down_kernel<BITS, false><<<gblocks, 256, 0, s>>>(
-            o, poffset, pcount, n_groups, size, local, pre, crit, theta, rec64, rec32,
-            box, need_center, eps, w, sum4, h);
+            o, poffset, pcount, n_groups, la, rec64, rec32, box, need_center, eps, w, horder);
     NBH_HIP(hipGetLastError());
 }
-
-size_t level_info_ints() { return kLevelInfoInts; }
 
 // Writes the error sentinel "error 4, skip to `cap`" into every link of
 // freshly allocated walk records.  Records are later overwritten only by
@@ -920,30 +843,27 @@ void launch_walk_sentinel(hipStream_t s, int4* w32node, Walk64Rec* w64, int cap)
 
 // The reference-layout per-node arrays of the pipeline's octree (mass and
 // centre of mass, cell centre and half size, child counts), filled from the
-// packed sums on request: the level passes themselves only keep sum4.
+// level passes' packed sums on request: the passes themselves keep only those.
 template <int BITS>
-void launch_octree_views(hipStream_t s, const nbh_octree_view& o,
-                         const double4* sum4, const double* box)
+void launch_octree_views(hipStream_t s, const nbh_octree_view& o, const LevelArrays& la,
+                         const double* box)
 {
-    octree_views_kernel<BITS><<<blocks_for(o.n_nodes, 256), 256, 0, s>>>(o, sum4, box);
+    octree_views_kernel<BITS><<<blocks_for(o.n_nodes, 256), 256, 0, s>>>(o, la, box);
     NBH_HIP(hipGetLastError());
 }
 template void launch_octree_views<10>(hipStream_t, const nbh_octree_view&,
-                                      const double4*, const double*);
+                                      const LevelArrays&, const double*);
 template void launch_octree_views<21>(hipStream_t, const nbh_octree_view&,
-                                      const double4*, const double*);
+                                      const LevelArrays&, const double*);
 
-#define NBH_LEVEL_INST(B)                                                            \
-    template void launch_level_passes<B>(hipStream_t, const nbh_octree_view&, int, bool, \
-                                         const double4*, const int4*, double,           \
-                                         const double*, bool, double, int*, int*,       \
-                                         double*, int*, int*, int*, double4*,           \
-                                         const int*, const int*, int, double, Walk64Rec*, \
-                                         double4*, int4*, int4*, int4*, int*, hipEvent_t, \
-                                         int*, int*, uint32_t*, const uint8_t*);
+#define NBH_LEVEL_INST(B)                                                               \
+    template void launch_level_passes<B>(                                               \
+        hipStream_t, const nbh_octree_view&, const LevelStarts&, bool, const double4*,  \
+        const int4*, double, const double*, bool, double, const LevelArrays&,           \
+        const int*, const int*, const int*, int, double, Walk64Rec*, double4*, int4*,   \
+        int4*, int*, hipEvent_t, uint32_t*, const uint8_t*);
 NBH_LEVEL_INST(10)
 NBH_LEVEL_INST(21)
 #undef NBH_LEVEL_INST
-
 
 } // namespace nbh

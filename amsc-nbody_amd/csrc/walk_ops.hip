@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
 // particle k of a leaf v adds m_k u_v to its position cotangent and
 // w_v + x_k . u_v to its mass cotangent.
 __global__ __launch_bounds__(256) void push_down_kernel(
-    nbh_octree_view o, int L, const int* __restrict__ pre_of, int wcount,
+    nbh_octree_view o, int L, const int2* __restrict__ pos, int wcount,
     const double* __restrict__ nbx, const double* __restrict__ nby,
     const double* __restrict__ nbz, const double* __restrict__ nm,
     double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz,
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) void push_down_kernel(
             uz = sz[p];
             uw = sw[p];
         }
-        const int pre = pre_of[v];
+        const int pre = pos[v].x;
         const double m = o.mass[v];
         if (m > 0.0 && pre >= 0 && pre < wcount) {
             const double bx = nbx[pre] / m;
@@ -374,7 +374,7 @@ void launch_energy_partials(hipStream_t s, const double* m, const double* vx,
 }
 
 void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
-                           int max_level, const int* pre_of,
+                           int max_level, const int2* pos,
                            const Walk64Rec* w64, const double4* wcenter,
                            const int* d_wcount, int wcount_host,
                            const double4* rec, const uint32_t* idx, int n,
@@ -417,7 +417,7 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
         blocks = 2048;
     for (int L = 0; L <= max_level; ++L) {
         push_down_kernel<<<blocks, 256, 0, s>>>(
-            o, L, pre_of, wcount_host, a.nbx, a.nby, a.nbz, a.nm, sv, sv + M,
+            o, L, pos, wcount_host, a.nbx, a.nby, a.nbz, a.nm, sv, sv + M,
             sv + 2 * M, sv + 3 * M, rec, a.pbx, a.pby, a.pbz, a.pm);
         NBH_HIP(hipGetLastError());
     }
