@@ -120,6 +120,9 @@ void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scr
     const double eps = s->cfg.softening;
     sim_fill_views(s); // the push-down reads node masses
     nbh_octree_view o = s->octree_view();
+    // and every node's walk position, leaves included
+    launch_leaf_positions(s->stream, o, s->poffset.p, s->pcount.p, s->ngroups,
+                          s->level_arrays());
     launch_force_vjp_fp64(s->stream, o, s->cfg.bits_per_axis, s->pos.p,
                           s->w64.p, s->w64center.p, s->scalars.p + 3, static_cast<int>(M),
                           s->rec64.p, s->idx.p, static_cast<int>(n), eps * eps,

@@ -37,6 +37,10 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, const LevelSta
                          const int* pcount, int n_groups, double G, Walk64Rec* w64,
                          double4* w64center, int4* w32node, int4* w32center, int* wcount,
                          hipEvent_t between, uint32_t* horder, const uint8_t* htab);
+// The leaves' {pre, curve offset} words of the last layout (the layout
+// passes write them for the internal nodes only).
+void launch_leaf_positions(hipStream_t s, const nbh_octree_view& o, const int* poffset,
+                           const int* pcount, int n_groups, const LevelArrays& la);
 void launch_walk_sentinel(hipStream_t s, int4* w32node, Walk64Rec* w64, int cap);
 template <int BITS>
 void launch_octree_views(hipStream_t s, const nbh_octree_view& o, const LevelArrays& la,

@@ -473,8 +473,11 @@ __global__ __launch_bounds__(256) void level_down_kernel(
             else
                 w.w64[wc] = error_record64(wc);
         } else {
+            const int parent = o.parent[v];
+            if (parent < 0 || parent >= o.n_nodes)
+                continue;
             const int2 lk = la.link[v];
-            const int2 p = la.pos[o.parent[v]];
+            const int2 p = la.pos[parent];
             pre = p.x + lk.x;
             hoff = p.y + lk.y;
         }
@@ -649,7 +652,6 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
     const int pc = o.particle_count[v];
     const int pre = p.x + lk.x;
     const int off = p.y + lk.y;
-    la.pos[v] = make_int2(pre, off);
     if (horder) {
         // the leaf's particles, in sorted order, at its curve offset (its
         // parent's offset is absolute once every internal level is done)
@@ -660,7 +662,8 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
     if (pre < 0 || pre >= o.n_nodes)
         return;
     NodeVals nv;
-    nv.mc = la.sum4[v];
+    // a one-particle leaf's record is its particle's: no sums needed
+    nv.mc = pc > 1 ? la.sum4[v] : make_double4(0.0, 0.0, 0.0, 0.0);
     nv.geom = make_double4(0.0, 0.0, 0.0, 0.0);
     nv.crit = 0.0;
     nv.kind = KIND_LEAF;
@@ -674,6 +677,27 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
                             nv.geom.w, code);
     }
     emit_record<MIXED>(nv, pre, rec64, rec32, box, need_center, eps, w);
+}
+
+// Leaf positions {pre, curve offset} for the callers that read them per
+// node (the force VJP's push-down); the layout passes keep them for the
+// internal nodes only.
+__global__ __launch_bounds__(256) void leaf_positions_kernel(
+    nbh_octree_view o, const int* __restrict__ poffset, const int* __restrict__ pcount,
+    int n_groups, LevelArrays la)
+{
+    const int g = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= n_groups)
+        return;
+    const int v = leaf_of_group(poffset, pcount, o, g);
+    if (v < 0)
+        return;
+    const int parent = o.parent[v];
+    if (parent < 0 || parent >= o.n_nodes)
+        return;
+    const int2 lk = la.link[v];
+    const int2 p = la.pos[parent];
+    la.pos[v] = make_int2(p.x + lk.x, p.y + lk.y);
 }
 
 // Every link starts as "error 4, skip to the end", so a position that the
@@ -752,6 +776,14 @@ void launch_level_scatter(hipStream_t s, const int* d_nleaves, int n_leaves,
     level_scatter_kernel<<<blocks, kScatterThreads, 0, s>>>(d_nleaves, plevel, pcount,
                                                             poffset, level_count, cursor,
                                                             order);
+    NBH_HIP(hipGetLastError());
+}
+
+void launch_leaf_positions(hipStream_t s, const nbh_octree_view& o, const int* poffset,
+                           const int* pcount, int n_groups, const LevelArrays& la)
+{
+    leaf_positions_kernel<<<blocks_for(n_groups > 0 ? n_groups : 1, 256), 256, 0, s>>>(
+        o, poffset, pcount, n_groups, la);
     NBH_HIP(hipGetLastError());
 }
 
