@@ -233,9 +233,7 @@ void launch_octree_build(hipStream_t s, const nbh_octree_view& o,
                          const typename Layout<BITS>::Key* unique,
                          const int* gfirst, const int* gcount,
                          const int* plevel, const int* pcount,
-                         const int* poffset, int* err, bool count_children,
-                         const double4* rec64 = nullptr, double4* sum4 = nullptr,
-                         int4* aux = nullptr);
+                         const int* poffset, int* err, bool count_children);
 template <int BITS>
 void launch_geometry(hipStream_t s, const nbh_octree_view& o, double cx,
                      double cy, double cz, double side, int* err);
@@ -269,12 +267,12 @@ struct alignas(64) Walk64Rec {
 static_assert(sizeof(Walk64Rec) == 64, "FP64 walk records are 64 bytes");
 
 // Per-node state of the pipeline's level passes (one entry per octree
-// node in each array):
+// node in each array; sum4, aux and crit for internal nodes only: a leaf's
+// values follow from its particles):
 //   sum4  {com x, y, z, mass}        written bottom-up by the node
-//   aux   {size, hcount, pc, 0}      bottom-up by the node: walk positions
-//                                    of the subtree, its particle count, a
-//                                    leaf's own particle count (0 internal)
-//   crit  opening radius             bottom-up by an internal node
+//   aux   {size, hcount, 0, 0}       bottom-up by the node: walk positions
+//                                    of the subtree and its particle count
+//   crit  opening radius             bottom-up by the node
 //   link  {local, hoffset}           bottom-up by the parent: position and
 //                                    Hilbert-curve offsets inside the
 //                                    parent's subtree

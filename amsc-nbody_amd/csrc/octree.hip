@@ -49,9 +49,7 @@ __global__ __launch_bounds__(256) void materialize_kernel(
     const typename Layout<BITS>::Key* __restrict__ unique,
     const int* __restrict__ gfirst, const int* __restrict__ gcount,
     const int* __restrict__ plevel, const int* __restrict__ pcount,
-    const int* __restrict__ poffset, int* err, int* child_count,
-    const double4* __restrict__ rec64, double4* __restrict__ sum4,
-    int4* __restrict__ aux)
+    const int* __restrict__ poffset, int* err, int* child_count)
 {
     using L = Layout<BITS>;
     using Key = typename L::Key;
@@ -146,32 +144,8 @@ __global__ __launch_bounds__(256) void materialize_kernel(
             raise_error(err, 5);
             return;
         }
-        const int f = gfirst[node];
-        const int pc = gcount[node];
-        o.first_particle[leaf] = f;
-        o.particle_count[leaf] = pc;
-        if (sum4) {
-            // The pipeline's leaf sums for its level passes: mass and
-            // centre of mass over the leaf's particles in sorted order
-            // (octree_physics.cu:94-153), subtree size 1.
-            double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
-            for (int k = 0; k < pc; ++k) {
-                const double4 p = rec64[f + k];
-                tm += p.w;
-                wx = fma(p.w, p.x, wx);
-                wy = fma(p.w, p.y, wy);
-                wz = fma(p.w, p.z, wz);
-            }
-            double cx = 0.0, cy = 0.0, cz = 0.0;
-            if (tm > 0.0) {
-                const double inv = 1.0 / tm;
-                cx = wx * inv;
-                cy = wy * inv;
-                cz = wz * inv;
-            }
-            sum4[leaf] = make_double4(cx, cy, cz, tm);
-            aux[leaf] = make_int4(1, pc, pc, 0);
-        }
+        o.first_particle[leaf] = gfirst[node];
+        o.particle_count[leaf] = gcount[node];
     }
 }
 
@@ -335,8 +309,7 @@ void launch_octree_build(hipStream_t s, const nbh_octree_view& o,
                          const typename Layout<BITS>::Key* unique,
                          const int* gfirst, const int* gcount,
                          const int* plevel, const int* pcount,
-                         const int* poffset, int* err, bool count_children,
-                         const double4* rec64, double4* sum4, int4* aux)
+                         const int* poffset, int* err, bool count_children)
 {
     unsigned blocks = blocks_for(o.n_nodes, 256);
     if (blocks > 8192)
@@ -348,7 +321,7 @@ void launch_octree_build(hipStream_t s, const nbh_octree_view& o,
     const int R = 2 * n_leaves - 1;
     materialize_kernel<BITS><<<blocks_for(R, 256), 256, 0, s>>>(
         o, n_leaves, tparent, range_first, unique, gfirst, gcount, plevel,
-        pcount, poffset, err, count_children ? o.child_count : nullptr, rec64, sum4, aux);
+        pcount, poffset, err, count_children ? o.child_count : nullptr);
     NBH_HIP(hipGetLastError());
 }
 
@@ -387,8 +360,7 @@ void launch_propagate(hipStream_t s, const nbh_octree_view& o, int* err)
                                          const int*, const int*,                     \
                                          const typename Layout<B>::Key*, const int*, \
                                          const int*, const int*, const int*,         \
-                                         const int*, int*, bool, const double4*,     \
-                                         double4*, int4*);                           \
+                                         const int*, int*, bool);                    \
     template void launch_geometry<B>(hipStream_t, const nbh_octree_view&, double,    \
                                      double, double, double, int*);
 NBH_OCTREE_INST(10)

@@ -27,8 +27,7 @@ void launch_level_scatter(hipStream_t s, const int* d_nleaves, int n_leaves,
                           const int* plevel, const int* pcount, const int* poffset,
                           const int* level_count, int* cursor, int* order);
 // Mass / COM, subtree sizes and the walk layout over the level-grouped
-// internal nodes and the leaves (one per Morton group); the leaves' sums and
-// aux words come from the octree materialisation.
+// internal nodes and the leaves (one per Morton group).
 template <int BITS>
 void launch_level_passes(hipStream_t s, const nbh_octree_view& o, const LevelStarts& st,
                          bool mixed, const double4* rec64, const int4* rec32, double theta,
@@ -44,7 +43,7 @@ void launch_leaf_positions(hipStream_t s, const nbh_octree_view& o, const int* p
 void launch_walk_sentinel(hipStream_t s, int4* w32node, Walk64Rec* w64, int cap);
 template <int BITS>
 void launch_octree_views(hipStream_t s, const nbh_octree_view& o, const LevelArrays& la,
-                         const double* box);
+                         const double4* rec64, const double* box);
 
 template <typename T>
 struct DevBuf {

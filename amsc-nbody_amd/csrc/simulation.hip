@@ -217,7 +217,7 @@ void rebuild(nbh_sim* s, bool kick_drift)
     launch_octree_build<BITS>(st, o, s->ngroups, s->tparent.p, s->rfirst.p,
                               unique, s->gfirst.p, s->gcount.p, s->plevel.p,
                               s->pcount.p, s->poffset.p, s->errors + ERR_OCTREE,
-                              false, s->rec64.p, la.sum4, la.aux);
+                              false);
     launch_level_scatter(st, d_ng, s->ngroups, s->plevel.p, s->pcount.p, s->poffset.p,
                          s->mid->level_count, s->mid->level_cursor, s->order.p);
     s->mark(PH_COM);
@@ -288,9 +288,9 @@ void sim_fill_views(nbh_sim* s)
         return;
     nbh_octree_view o = s->octree_view();
     if (s->cfg.bits_per_axis == 10)
-        launch_octree_views<10>(s->stream, o, s->level_arrays(), s->box);
+        launch_octree_views<10>(s->stream, o, s->level_arrays(), s->rec64.p, s->box);
     else
-        launch_octree_views<21>(s->stream, o, s->level_arrays(), s->box);
+        launch_octree_views<21>(s->stream, o, s->level_arrays(), s->rec64.p, s->box);
     s->views_fresh = true;
 }
 

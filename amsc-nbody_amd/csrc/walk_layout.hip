@@ -346,11 +346,40 @@ __global__ __launch_bounds__(256) void walk_build_kernel(
 // radius side/theta + |com - centre| (barnes_hut.cu:277-291); plus the
 // subtree size, the pre-order offsets of the children (octants 7 -> 0) and
 // the children's Hilbert-curve offsets.
+// A leaf's mass / centre of mass over its pc particles in sorted order
+// (octree_physics.cu:94-153), given its first particle p0 = rec[f].
+__device__ __forceinline__ double4 leaf_sum(double4 p0, const double4* __restrict__ rec,
+                                            int f, int pc)
+{
+    double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+    if (pc > 0) {
+        tm += p0.w;
+        wx = fma(p0.w, p0.x, wx);
+        wy = fma(p0.w, p0.y, wy);
+        wz = fma(p0.w, p0.z, wz);
+    }
+    for (int k = 1; k < pc; ++k) {
+        const double4 p = rec[f + k];
+        tm += p.w;
+        wx = fma(p.w, p.x, wx);
+        wy = fma(p.w, p.y, wy);
+        wz = fma(p.w, p.z, wz);
+    }
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    if (tm > 0.0) {
+        const double inv = 1.0 / tm;
+        cx = wx * inv;
+        cy = wy * inv;
+        cz = wz * inv;
+    }
+    return make_double4(cx, cy, cz, tm);
+}
+
 template <int BITS>
 __global__ __launch_bounds__(256) void level_up_kernel(
     nbh_octree_view o, int L, int begin, int end, const int* __restrict__ order,
-    LevelArrays la, const double* __restrict__ box, double theta, bool collapse,
-    const uint8_t* __restrict__ tab)
+    LevelArrays la, const double4* __restrict__ rec64, const double* __restrict__ box,
+    double theta, bool collapse, const uint8_t* __restrict__ tab)
 {
     using Key = typename Layout<BITS>::Key;
     __shared__ uint8_t s_tab[kHilbertTabBytes];
@@ -369,18 +398,42 @@ __global__ __launch_bounds__(256) void level_up_kernel(
         const int4 c1 =
             reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v) + 1];
         const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-        // every present child's words are requested before any is used
+        // Every present child's words are requested before any is used.
+        // Leaves are octree nodes 1 .. n_leaves (radix leaf g owns node
+        // g + 1): a leaf child's sums follow from its particles, an internal
+        // child's come from its own level pass.
         double4 cs[8];
-        int4 ca[8]; // size, hcount, pc
+        int4 ca[8]; // size, hcount, particle count
+        int lf[8];  // a leaf child's first particle
+#pragma unroll
+        for (int oc = 0; oc < 8; ++oc) {
+            ca[oc] = make_int4(0, 0, 0, 0);
+            lf[oc] = 0;
+            const int c = ch[oc];
+            if (c != -1 && c <= o.n_leaves) {
+                lf[oc] = o.first_particle[c];
+                const int pc = o.particle_count[c];
+                ca[oc] = make_int4(1, pc, pc, 0);
+            }
+        }
 #pragma unroll
         for (int oc = 0; oc < 8; ++oc) {
             cs[oc] = make_double4(0.0, 0.0, 0.0, 0.0);
-            ca[oc] = make_int4(0, 0, 0, 0);
-            if (ch[oc] != -1) {
-                cs[oc] = la.sum4[ch[oc]];
-                ca[oc] = la.aux[ch[oc]];
+            const int c = ch[oc];
+            if (c != -1) {
+                if (c <= o.n_leaves) {
+                    if (ca[oc].z > 0)
+                        cs[oc] = rec64[lf[oc]]; // the leaf's first particle
+                } else {
+                    cs[oc] = la.sum4[c];
+                    ca[oc] = la.aux[c];
+                }
             }
         }
+#pragma unroll
+        for (int oc = 0; oc < 8; ++oc)
+            if (ch[oc] != -1 && ch[oc] <= o.n_leaves)
+                cs[oc] = leaf_sum(cs[oc], rec64, lf[oc], ca[oc].z);
         double tm = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
         int nch = 0, only_pc = 0; // child count, the last child's particle count
 #pragma unroll
@@ -583,7 +636,8 @@ __global__ __launch_bounds__(kScatterThreads) void level_scatter_kernel(
 
 template <int BITS>
 __global__ __launch_bounds__(256) void octree_views_kernel(
-    nbh_octree_view o, LevelArrays la, const double* __restrict__ box)
+    nbh_octree_view o, LevelArrays la, const double4* __restrict__ rec64,
+    const double* __restrict__ box)
 {
     using Key = typename Layout<BITS>::Key;
     const int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -598,13 +652,16 @@ __global__ __launch_bounds__(256) void octree_views_kernel(
     o.center_y[v] = gy;
     o.center_z[v] = gz;
     o.half_size[v] = gh;
-    const double4 mc = la.sum4[v];
+    const int pc = o.particle_count[v];
+    const double4 mc = pc > 0 ? leaf_sum(rec64[o.first_particle[v]], rec64,
+                                         o.first_particle[v], pc)
+                              : la.sum4[v];
     o.mass[v] = mc.w;
     o.com_x[v] = mc.x;
     o.com_y[v] = mc.y;
     o.com_z[v] = mc.z;
     int nch = 0;
-    if (o.particle_count[v] <= 0)
+    if (pc <= 0)
         for (int oc = 0; oc < 8; ++oc)
             nch += o.children[8 * static_cast<size_t>(v) + oc] != -1;
     o.child_count[v] = nch;
@@ -617,7 +674,7 @@ __global__ __launch_bounds__(256) void octree_views_kernel(
 // leaf g, node edgeNodeOffset[g] + edgeNodeCount[g] (octree numbering of
 // octree_builder.cu:49-224).  Groups, their particles and their leaves are
 // all in Morton order, so these passes read and write nearly sequentially.
-// The leaves' sums are written by the octree materialisation.
+// A leaf's sums are formed from its particles where they are needed.
 __device__ __forceinline__ int leaf_of_group(const int* __restrict__ poffset,
                                              const int* __restrict__ pcount,
                                              const nbh_octree_view& o, int g)
@@ -663,7 +720,7 @@ __global__ __launch_bounds__(256) void leaf_down_kernel(
         return;
     NodeVals nv;
     // a one-particle leaf's record is its particle's: no sums needed
-    nv.mc = pc > 1 ? la.sum4[v] : make_double4(0.0, 0.0, 0.0, 0.0);
+    nv.mc = pc > 1 ? leaf_sum(rec64[f], rec64, f, pc) : make_double4(0.0, 0.0, 0.0, 0.0);
     nv.geom = make_double4(0.0, 0.0, 0.0, 0.0);
     nv.crit = 0.0;
     nv.kind = KIND_LEAF;
@@ -788,7 +845,7 @@ void launch_leaf_positions(hipStream_t s, const nbh_octree_view& o, const int* p
 }
 
 // The pipeline's mass/COM + walk layout: the internal levels max_level..0
-// bottom-up (the leaves' sums come from the octree materialisation), then
+// bottom-up (each leaf's sums are formed by its parent from its particles), then
 // 0..max_level top-down, then the leaves' records; one launch per level,
 // sized to its node count.
 template <int BITS>
@@ -819,7 +876,7 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, const LevelSta
         if (st.s[L + 1] == st.s[L])
             continue;
         level_up_kernel<BITS><<<grid(L), 256, 0, s>>>(o, L, st.s[L], st.s[L + 1], order, la,
-                                                      box, theta, collapse, htab);
+                                                      rec64, box, theta, collapse, htab);
         NBH_HIP(hipGetLastError());
     }
     if (between)
@@ -878,15 +935,15 @@ void launch_walk_sentinel(hipStream_t s, int4* w32node, Walk64Rec* w64, int cap)
 // level passes' packed sums on request: the passes themselves keep only those.
 template <int BITS>
 void launch_octree_views(hipStream_t s, const nbh_octree_view& o, const LevelArrays& la,
-                         const double* box)
+                         const double4* rec64, const double* box)
 {
-    octree_views_kernel<BITS><<<blocks_for(o.n_nodes, 256), 256, 0, s>>>(o, la, box);
+    octree_views_kernel<BITS><<<blocks_for(o.n_nodes, 256), 256, 0, s>>>(o, la, rec64, box);
     NBH_HIP(hipGetLastError());
 }
 template void launch_octree_views<10>(hipStream_t, const nbh_octree_view&,
-                                      const LevelArrays&, const double*);
+                                      const LevelArrays&, const double4*, const double*);
 template void launch_octree_views<21>(hipStream_t, const nbh_octree_view&,
-                                      const LevelArrays&, const double*);
+                                      const LevelArrays&, const double4*, const double*);
 
 #define NBH_LEVEL_INST(B)                                                               \
     template void launch_level_passes<B>(                                               \
