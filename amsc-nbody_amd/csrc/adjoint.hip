@@ -44,6 +44,15 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
                            const double* abx, const double* aby,
                            const double* abz, double* scratch, double* gx,
                            double* gy, double* gz, double* gm, int* err);
+void launch_force_vjp_mixed(hipStream_t s, const nbh_octree_view& o, int max_level,
+                            const int2* pos, const int4* aux, int4* wrec,
+                            const int4* wcenter, const int* d_wcount, int* split,
+                            int wcount_host, const int4* rec32, const double4* rec64,
+                            const uint32_t* idx, const uint32_t* order, int n, float eps2,
+                            double gunit, double munit, bool contains, bool soft,
+                            const double* abx, const double* aby, const double* abz,
+                            double* scratch, double* gx, double* gy, double* gz,
+                            double* gm, int* err);
 size_t force_vjp_scratch_doubles(size_t n, size_t wcount, size_t n_nodes);
 
 namespace {
@@ -123,12 +132,27 @@ void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scr
     // and every node's walk position, leaves included
     launch_leaf_positions(s->stream, o, s->poffset.p, s->pcount.p, s->ngroups,
                           s->level_arrays());
-    launch_force_vjp_fp64(s->stream, o, s->cfg.bits_per_axis, s->pos.p,
-                          s->w64.p, s->w64center.p, s->scalars.p + 3, static_cast<int>(M),
-                          s->rec64.p, s->idx.p, static_cast<int>(n), eps * eps,
-                          s->cfg.gravitational_constant, contains, abar,
-                          abar + n, abar + 2 * n, scratch.p, xbar, xbar + n,
-                          xbar + 2 * n, mbar, s->errors + ERR_TRAVERSAL);
+    if (mixed(s)) {
+        // the mixed records' units: fixed-point steps per length, m_max
+        const double scale = 2147483648.0 / s->rb->box[3];
+        const double G = s->cfg.gravitational_constant;
+        launch_force_vjp_mixed(
+            s->stream, o, s->cfg.bits_per_axis, s->pos.p, s->aux.p, s->w32node.p,
+            s->w32center.p, s->scalars.p + 3,
+            walk_mixed_splits(static_cast<int>(n)) ? s->wsplit.p : nullptr,
+            static_cast<int>(M), s->rec32.p, s->rec64.p, s->idx.p, s->horder.p,
+            static_cast<int>(n), static_cast<float>(mixed_eps2(eps, scale)),
+            G * s->rb->box[4] * scale * scale * scale, G * scale * scale, contains, eps > 0.0,
+            abar, abar + n, abar + 2 * n, scratch.p, xbar, xbar + n, xbar + 2 * n, mbar,
+            s->errors + ERR_TRAVERSAL);
+    } else {
+        launch_force_vjp_fp64(s->stream, o, s->cfg.bits_per_axis, s->pos.p,
+                              s->w64.p, s->w64center.p, s->scalars.p + 3, static_cast<int>(M),
+                              s->rec64.p, s->idx.p, static_cast<int>(n), eps * eps,
+                              s->cfg.gravitational_constant, contains, abar,
+                              abar + n, abar + 2 * n, scratch.p, xbar, xbar + n,
+                              xbar + 2 * n, mbar, s->errors + ERR_TRAVERSAL);
+    }
     s->mark(PH_KICK);
     s->mark(PH_COUNT);
     finish(s);
@@ -258,7 +282,8 @@ int nbh_sim_force_vjp(nbh_sim* s, const double* d_abx, const double* d_aby,
     return guard([&] {
         if (!s || !d_abx || !d_aby || !d_abz || !d_gx || !d_gy || !d_gz)
             fail(NBH_INVALID_ARGUMENT, "arguments must not be null");
-        require_fp64_walk(s, "The force vector-Jacobian product");
+        if (s->state != 1)
+            fail(NBH_LOGIC_ERROR, "Simulation must be initialized");
         const size_t n = s->n;
         DevBuf<double> ab, g, scratch;
         ab.alloc(3 * n);
@@ -286,8 +311,9 @@ int nbh_adjoint_create(nbh_adjoint** out, const nbh_sim_config* config,
             fail(NBH_INVALID_ARGUMENT, "output pointer must not be null");
         if (!config)
             fail(NBH_INVALID_ARGUMENT, "config must not be null");
-        if (config->precision != NBH_PRECISION_FP64)
-            fail(NBH_INVALID_ARGUMENT, "the adjoint runs the FP64 pipeline");
+        if (config->precision != NBH_PRECISION_FP64 &&
+            config->precision != NBH_PRECISION_MIXED)
+            fail(NBH_INVALID_ARGUMENT, "unknown precision");
         if (steps < 1)
             fail(NBH_INVALID_ARGUMENT, "the adjoint needs at least one step");
         if (checkpoint_every < 1)

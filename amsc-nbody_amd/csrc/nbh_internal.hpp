@@ -270,8 +270,10 @@ static_assert(sizeof(Walk64Rec) == 64, "FP64 walk records are 64 bytes");
 // node in each array; sum4, aux and crit for internal nodes only: a leaf's
 // values follow from its particles):
 //   sum4  {com x, y, z, mass}        written bottom-up by the node
-//   aux   {size, hcount, 0, 0}       bottom-up by the node: walk positions
-//                                    of the subtree and its particle count
+//   aux   {size, hcount, 0, c}       bottom-up by the node: walk positions
+//                                    of the subtree, its particle count, and
+//                                    c = 1 when the mixed layout collapsed
+//                                    it into its single child
 //   crit  opening radius             bottom-up by the node
 //   link  {local, hoffset}           bottom-up by the parent: position and
 //                                    Hilbert-curve offsets inside the
@@ -320,6 +322,10 @@ constexpr int kWalkSplitInts = 1 + 4 * kWalkSplitMax;
 // Whether the mixed traversal of n targets is split (a function of n only,
 // so that results do not depend on the device).
 bool walk_mixed_splits(int n);
+// Marks the subtree starts of a split mixed walk below its first record and
+// fills the split table (kWalkSplitInts ints); a layout that does not split
+// gets a one-unit table covering the whole walk.
+void launch_walk_split(hipStream_t s, int4* wpairs, const int* d_wcount, int* split);
 // wcap: records the wpairs buffer holds.
 // order: sorted target indices in the packets' order (null: Morton order).
 // wpairs: the {node, link} record pairs; a split walk marks its subtrees'

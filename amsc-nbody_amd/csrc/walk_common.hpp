@@ -22,6 +22,10 @@ enum : int {
 // particles is written as an error record of this traversal code.
 constexpr int kMixedLeafCountLimit = 1 << 23;
 constexpr int kErrMixedLeafTooLarge = 9;
+// Kind-byte flag of the mixed records that start a split walk's subtrees
+// (crit^2 = +inf while marked; their own crit^2 and kind word are kept in
+// the split table).
+constexpr int kMixedBoundary = 0x80;
 
 
 // "Error 4 (inconsistent topology), skip to `skip`": the FP64 walk's

@@ -488,7 +488,8 @@ __global__ __launch_bounds__(256) void level_up_kernel(
         const double oz = cz - gz;
         const double o2 = fma(oz, oz, fma(ox, ox, oy * oy));
         la.sum4[v] = make_double4(cx, cy, cz, tm);
-        la.aux[v] = make_int4(off, hc, 0, 0);
+        // aux.w: collapsed into its child (no walk position of its own)
+        la.aux[v] = make_int4(off, hc, 0, (collapse && nch == 1 && only_pc <= 1) ? 1 : 0);
         la.crit[v] = (2.0 * gh) / theta + sqrt(o2);
     }
 }

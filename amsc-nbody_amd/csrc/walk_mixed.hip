@@ -43,8 +43,7 @@ namespace nbh {
 namespace {
 
 constexpr int kFold = 64;
-// Kind-byte flag of the records that start a split walk's subtrees.
-constexpr int kBoundary = 0x80;
+constexpr int kBoundary = kMixedBoundary;
 
 // One position of the mixed walk: the node (fixed-point COM, m / m_max) and
 // its link (crit^2 + eps^2, skip, kind | count << 8, first particle).
@@ -775,6 +774,12 @@ __global__ __launch_bounds__(256) void walk_split_sum_kernel(
 bool walk_mixed_splits(int n)
 {
     return n > 0 && (n + 63) / 64 <= kWalkSplitPackets;
+}
+
+void launch_walk_split(hipStream_t s, int4* wpairs, const int* d_wcount, int* split)
+{
+    walk_split_kernel<<<1, 64, 0, s>>>(reinterpret_cast<WalkRec*>(wpairs), d_wcount, split);
+    NBH_HIP(hipGetLastError());
 }
 
 void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,

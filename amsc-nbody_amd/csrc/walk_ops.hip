@@ -226,7 +226,8 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
 // particle k of a leaf v adds m_k u_v to its position cotangent and
 // w_v + x_k . u_v to its mass cotangent.
 __global__ __launch_bounds__(256) void push_down_kernel(
-    nbh_octree_view o, int L, const int2* __restrict__ pos, int wcount,
+    nbh_octree_view o, int L, const int2* __restrict__ pos,
+    const int4* __restrict__ aux, int wcount,
     const double* __restrict__ nbx, const double* __restrict__ nby,
     const double* __restrict__ nbz, const double* __restrict__ nm,
     double* __restrict__ sx, double* __restrict__ sy, double* __restrict__ sz,
@@ -249,7 +250,10 @@ __global__ __launch_bounds__(256) void push_down_kernel(
         }
         const int pre = pos[v].x;
         const double m = o.mass[v];
-        if (m > 0.0 && pre >= 0 && pre < wcount) {
+        // a node the mixed layout collapsed into its single child owns no
+        // walk position (its child's record took it)
+        const bool owns = !(aux && o.particle_count[v] <= 0 && aux[v].w != 0);
+        if (owns && m > 0.0 && pre >= 0 && pre < wcount) {
             const double bx = nbx[pre] / m;
             const double by = nby[pre] / m;
             const double bz = nbz[pre] / m;
@@ -331,6 +335,278 @@ __global__ __launch_bounds__(256) void energy_partials_kernel(
         partials[2 * blockIdx.x] = sk[0];
         partials[2 * blockIdx.x + 1] = sp[0];
     }
+}
+
+// ------------------------------------------------ mixed-precision VJP
+// The force VJP along the mixed walk: the same records, opening test and
+// interaction lists as the mixed force traversal (fixed-point separations,
+// FP32 interactions), so the gradient belongs to the forces the mixed step
+// integrates.  Per interaction, in the records' units (separations d in
+// fixed-point units, source mass gm = m / m_max):
+//   g  = gm (r^-3 ab - 3 r^-5 d (d . ab)),   cm = (d . ab) r^-3,
+// with the target's abar scaled by a power of two common to the wave, so
+// that the FP32 terms stay in range and the wave's source sums need no
+// rescaling.  Physical units: x cotangents G m_max unit^3 g, mass
+// cotangents G unit^2 cm (unit = fixed-point steps per length).  Target
+// terms are summed per lane in FP32 partials folded into Kahan sums every
+// 64 visits; each source's terms are summed over the wave in FP32 and added
+// to FP64 accumulators (particles by Morton index, nodes by walk position).
+struct MixedVjp {
+    const int4* wrec;    // interleaved {node, link} pairs, 2 int4 per position
+    const int4* wcenter; // cell centres (theta >= 1.15), else null
+    const int* d_wcount;
+    const int* split;    // split table of the walk's units, else null
+    const int4* rec;     // particles, Morton order: fixed point, m / m_max
+    const uint32_t* idx; // Morton -> original index
+    const uint32_t* order; // target order, null for Morton order
+    int n;
+    int units;           // units per packet: kWalkSplitMax with a split table, else 1
+    float eps2;          // softening^2 in fixed-point units
+    double gunit;        // G m_max unit^3
+    double munit;        // G unit^2
+};
+
+// Sum over the wave's 64 lanes, complete in lane 63, by DPP moves within
+// and across rows of 16 lanes (no LDS traffic): an inclusive prefix sum in
+// each row (row_shr 1, 2, 4, 8), then row 0's and row 2's totals broadcast
+// into rows 1 and 3 (row_bcast 15) and the first half's total into rows 2
+// and 3 (row_bcast 31).  Lanes whose DPP source lies outside the row (or
+// rows outside the mask) add zero.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xF, false));
+}
+
+__device__ __forceinline__ float wave_sum_lane63(float v)
+{
+    v += dpp_f<0x111, 0xF>(v); // row_shr:1
+    v += dpp_f<0x112, 0xF>(v); // row_shr:2
+    v += dpp_f<0x114, 0xF>(v); // row_shr:4
+    v += dpp_f<0x118, 0xF>(v); // row_shr:8
+    v += dpp_f<0x142, 0xA>(v); // row_bcast:15 into rows 1 and 3
+    v += dpp_f<0x143, 0xC>(v); // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+struct VjpKahan {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float v)
+    {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const
+    {
+        return static_cast<double>(s) - static_cast<double>(c);
+    }
+};
+
+struct VjpLane {
+    int4 tq;
+    int t;
+    float bx, by, bz;       // abar / 2^e (e common to the wave)
+    float px = 0.0f, py = 0.0f, pz = 0.0f; // target partials (-g)
+    VjpKahan kx, ky, kz;
+    int code = 0;
+
+    // One source for the whole wave (use false on lanes that do not
+    // interact); wave-uniform control flow.
+    template <bool SOFT>
+    __device__ __forceinline__ void source(bool use, const int4& sq, float eps2,
+                                           double unit_g, double unit_m, double* out_x,
+                                           double* out_y, double* out_z, double* out_m,
+                                           int index)
+    {
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f, cm = 0.0f;
+        if (use) {
+            const float dx = static_cast<float>(sq.x - tq.x);
+            const float dy = static_cast<float>(sq.y - tq.y);
+            const float dz = static_cast<float>(sq.z - tq.z);
+            const float r2 = fmaf(dz, dz, fmaf(dx, dx, fmaf(dy, dy, eps2)));
+            const float gm = __int_as_float(sq.w);
+            if (!SOFT && r2 == 0.0f) {
+                if (gm != 0.0f)
+                    code = 6;
+            } else {
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float inv2 = inv * inv;
+                const float inv3 = inv2 * inv;
+                const float dot = fmaf(dz, bz, fmaf(dx, bx, dy * by));
+                const float t1 = gm * inv3;
+                const float t2 = 3.0f * t1 * dot * inv2;
+                gx = fmaf(t1, bx, -t2 * dx);
+                gy = fmaf(t1, by, -t2 * dy);
+                gz = fmaf(t1, bz, -t2 * dz);
+                cm = dot * inv3;
+                px -= gx;
+                py -= gy;
+                pz -= gz;
+            }
+        }
+        if (__ballot(gx != 0.0f || gy != 0.0f || gz != 0.0f || cm != 0.0f) != 0ull) {
+            gx = wave_sum_lane63(gx);
+            gy = wave_sum_lane63(gy);
+            gz = wave_sum_lane63(gz);
+            cm = wave_sum_lane63(cm);
+            if ((threadIdx.x & 63) == 63) {
+                atomicAdd(out_x + index, static_cast<double>(gx) * unit_g);
+                atomicAdd(out_y + index, static_cast<double>(gy) * unit_g);
+                atomicAdd(out_z + index, static_cast<double>(gz) * unit_g);
+                atomicAdd(out_m + index, static_cast<double>(cm) * unit_m);
+            }
+        }
+    }
+};
+
+// One unit per wave: a packet of 64 targets and, with a split table, one
+// subtree of the walk below its first record (which every target opens), so
+// that few packets still fill the device; target terms are added to FP64
+// accumulators per unit.
+template <bool CONTAINS, bool SOFT>
+__global__ __launch_bounds__(kWalkThreads) void vjp_mixed_kernel(MixedVjp m, OpArgs a,
+                                                                 int* err)
+{
+    const int lane = threadIdx.x & 63;
+    const int unit = static_cast<int>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int packet = unit / m.units;
+    const int sub = unit - packet * m.units;
+    const int n = m.n;
+    if (packet * 64 >= n)
+        return;
+    int begin = 0;
+    int end = *m.d_wcount;
+    if (m.split) {
+        if (sub >= m.split[0])
+            return;
+        begin = m.split[1 + 4 * sub];
+        end = m.split[2 + 4 * sub];
+    }
+    const int k = packet * 64 + lane;
+    bool valid = k < n;
+    VjpLane L;
+    L.t = k;
+    if (valid && m.order) {
+        const uint32_t o = m.order[k];
+        if (o < static_cast<uint32_t>(n)) {
+            L.t = static_cast<int>(o);
+        } else {
+            valid = false;
+            L.code = 3;
+        }
+    }
+    uint32_t orig = 0;
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    if (valid) {
+        orig = m.idx[L.t];
+        if (orig >= static_cast<uint32_t>(n)) {
+            L.code = 3;
+            valid = false;
+        } else {
+            bx = a.abx[orig];
+            by = a.aby[orig];
+            bz = a.abz[orig];
+        }
+    }
+    L.tq = valid ? m.rec[L.t] : make_int4(0, 0, 0, 0);
+    // a power of two at or above the wave's largest |abar| component
+    double bmax = fmax(fabs(bx), fmax(fabs(by), fabs(bz)));
+    for (int o = 32; o > 0; o >>= 1)
+        bmax = fmax(bmax, __shfl_xor(bmax, o));
+    int e = 0;
+    if (bmax > 0.0 && bmax <= 1.7e308)
+        (void)frexp(bmax, &e);
+    L.bx = static_cast<float>(ldexp(bx, -e));
+    L.by = static_cast<float>(ldexp(by, -e));
+    L.bz = static_cast<float>(ldexp(bz, -e));
+    const double ug = ldexp(m.gunit, e);
+    const double um = ldexp(m.munit, e);
+    int next = valid ? begin : 0x7FFFFFFF;
+    int i = begin;
+    unsigned cnt = 0;
+    while (i < end) {
+        const int4 nd = m.wrec[2 * static_cast<size_t>(i)];
+        int4 lk = m.wrec[2 * static_cast<size_t>(i) + 1];
+        if (m.split && (lk.z & kMixedBoundary) != 0) {
+            // a marked subtree start of a split layout: its own link words
+            const int subtrees = m.split[0];
+            for (int q = 0; q < subtrees; ++q)
+                if (m.split[1 + 4 * q] == i) {
+                    lk.x = m.split[3 + 4 * q];
+                    lk.z = m.split[4 + 4 * q];
+                }
+        }
+        const int skip = lk.y;
+        const int kind = lk.z & 0x7F;
+        const float crit2 = __int_as_float(lk.x);
+        const bool active = next <= i && L.code == 0;
+        bool open = false;
+        if (kind == KIND_INTERNAL && crit2 == -1.0f) {
+            // always taken: a one-particle leaf (count 1, its particle) or
+            // a zero-mass node (count 0, nothing)
+            if ((lk.z >> 8) == 1)
+                L.source<SOFT>(active && lk.w != L.t, nd, m.eps2, ug, um, a.pbx, a.pby,
+                               a.pbz, a.pm, lk.w);
+            if (active)
+                next = skip;
+        } else if (kind == KIND_INTERNAL) {
+            const float dx = static_cast<float>(nd.x - L.tq.x);
+            const float dy = static_cast<float>(nd.y - L.tq.y);
+            const float dz = static_cast<float>(nd.z - L.tq.z);
+            const float r2 = fmaf(dz, dz, fmaf(dx, dx, fmaf(dy, dy, m.eps2)));
+            bool accept = active && r2 > crit2;
+            if (CONTAINS && accept) {
+                const int4 c = m.wcenter[i];
+                accept = !(abs(L.tq.x - c.x) <= c.w && abs(L.tq.y - c.y) <= c.w &&
+                           abs(L.tq.z - c.z) <= c.w);
+            }
+            if (accept) {
+                next = skip;
+            } else if (active) {
+                open = true;
+                next = i + 1;
+            }
+            L.source<SOFT>(accept, nd, m.eps2, ug, um, a.nbx, a.nby, a.nbz, a.nm, i);
+        } else if (kind == KIND_LEAF) {
+            const int count = lk.z >> 8;
+            const int first = lk.w;
+            for (int q = first; q < first + count; ++q)
+                L.source<SOFT>(active && q != L.t, m.rec[q], m.eps2, ug, um, a.pbx, a.pby,
+                               a.pbz, a.pm, q);
+            if (active)
+                next = skip;
+        } else if (kind == KIND_SKIP) {
+            if (active)
+                next = skip;
+        } else if (active) {
+            L.code = (kind == KIND_CHILDLESS) ? 5 : kind - KIND_ERROR;
+        }
+        if (L.code != 0)
+            next = 0x7FFFFFFF;
+        if (++cnt == 64) {
+            L.kx.add(L.px);
+            L.ky.add(L.py);
+            L.kz.add(L.pz);
+            L.px = L.py = L.pz = 0.0f;
+            cnt = 0;
+        }
+        i = (__ballot(open) != 0ull) ? i + 1 : skip;
+    }
+    if (L.code != 0) {
+        raise_error(err, L.code);
+        return;
+    }
+    if (!valid)
+        return;
+    L.kx.add(L.px);
+    L.ky.add(L.py);
+    L.kz.add(L.pz);
+    atomicAdd(a.tbx + L.t, L.kx.value() * ug);
+    atomicAdd(a.tby + L.t, L.ky.value() * ug);
+    atomicAdd(a.tbz + L.t, L.kz.value() * ug);
 }
 
 template <int OP>
@@ -417,13 +693,85 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
         blocks = 2048;
     for (int L = 0; L <= max_level; ++L) {
         push_down_kernel<<<blocks, 256, 0, s>>>(
-            o, L, pos, wcount_host, a.nbx, a.nby, a.nbz, a.nm, sv, sv + M,
+            o, L, pos, nullptr, wcount_host, a.nbx, a.nby, a.nbz, a.nm, sv, sv + M,
             sv + 2 * M, sv + 3 * M, rec, a.pbx, a.pby, a.pbz, a.pm);
         NBH_HIP(hipGetLastError());
     }
     combine_kernel<<<blocks_for(N, 256), 256, 0, s>>>(idx, n, a.tbx, a.tby,
                                                      a.tbz, a.pbx, a.pby, a.pbz,
                                                      a.pm, gx, gy, gz, gm, err);
+    NBH_HIP(hipGetLastError());
+}
+
+// The mixed walk's force VJP: the VJP walk over the mixed records, then the
+// same FP64 push-down of the node cotangents and the same combination as
+// the FP64 product (nodes the layout collapsed into their child skipped).
+void launch_force_vjp_mixed(hipStream_t s, const nbh_octree_view& o, int max_level,
+                            const int2* pos, const int4* aux, int4* wrec,
+                            const int4* wcenter, const int* d_wcount, int* split,
+                            int wcount_host, const int4* rec32, const double4* rec64,
+                            const uint32_t* idx, const uint32_t* order, int n, float eps2,
+                            double gunit, double munit, bool contains, bool soft,
+                            const double* abx, const double* aby, const double* abz,
+                            double* scratch, double* gx, double* gy, double* gz,
+                            double* gm, int* err)
+{
+    const size_t N = static_cast<size_t>(n);
+    const size_t W = static_cast<size_t>(wcount_host);
+    const size_t M = static_cast<size_t>(o.n_nodes);
+    double* tb = scratch;
+    double* pb = tb + 3 * N;
+    double* pm = pb + 3 * N;
+    double* nbar = pm + N;
+    double* nm = nbar + 3 * W;
+    double* sv = nm + W;
+    launch_fill_f64(s, tb, 0.0, 7 * N + 4 * W);
+    OpArgs a{};
+    a.abx = abx;
+    a.aby = aby;
+    a.abz = abz;
+    a.tbx = tb;
+    a.tby = tb + N;
+    a.tbz = tb + 2 * N;
+    a.pbx = pb;
+    a.pby = pb + N;
+    a.pbz = pb + 2 * N;
+    a.nbx = nbar;
+    a.nby = nbar + W;
+    a.nbz = nbar + 2 * W;
+    a.pm = pm;
+    a.nm = nm;
+    // few packets: one unit per subtree below the first record
+    if (split)
+        launch_walk_split(s, wrec, d_wcount, split);
+    const int units = split ? kWalkSplitMax : 1;
+    const MixedVjp mv{wrec, wcenter, d_wcount, split, rec32, idx, order, n, units, eps2,
+                      gunit, munit};
+    const unsigned blocks = blocks_for((N + 63) / 64 * units, 4);
+    if (contains) {
+        if (soft)
+            vjp_mixed_kernel<true, true><<<blocks, kWalkThreads, 0, s>>>(mv, a, err);
+        else
+            vjp_mixed_kernel<true, false><<<blocks, kWalkThreads, 0, s>>>(mv, a, err);
+    } else {
+        if (soft)
+            vjp_mixed_kernel<false, true><<<blocks, kWalkThreads, 0, s>>>(mv, a, err);
+        else
+            vjp_mixed_kernel<false, false><<<blocks, kWalkThreads, 0, s>>>(mv, a, err);
+    }
+    NBH_HIP(hipGetLastError());
+    unsigned pblocks = blocks_for(M, 256);
+    if (pblocks > 2048)
+        pblocks = 2048;
+    for (int L = 0; L <= max_level; ++L) {
+        push_down_kernel<<<pblocks, 256, 0, s>>>(
+            o, L, pos, aux, wcount_host, a.nbx, a.nby, a.nbz, a.nm, sv, sv + M,
+            sv + 2 * M, sv + 3 * M, rec64, a.pbx, a.pby, a.pbz, a.pm);
+        NBH_HIP(hipGetLastError());
+    }
+    combine_kernel<<<blocks_for(N, 256), 256, 0, s>>>(idx, n, a.tbx, a.tby, a.tbz, a.pbx,
+                                                     a.pby, a.pbz, a.pm, gx, gy, gz, gm,
+                                                     err);
     NBH_HIP(hipGetLastError());
 }
 

@@ -1085,8 +1085,9 @@ class Simulation:
 
     def force_vjp(self, abx, aby, abz, with_mass=False):
         """(da/dx)^T abar at the current tree, and (da/dm)^T abar as a fourth
-        output with with_mass; device float64 tensors in, device tensors out
-        (FP64 simulations only)."""
+        output with with_mass; device float64 tensors in, device tensors out.
+        A mixed simulation differentiates its mixed forces (FP32 interactions
+        along the mixed walk, FP64 node push-down)."""
         torch = _torch()
         ab = [t.to(device=device(), dtype=torch.float64).contiguous() for t in (abx, aby, abz)]
         g = [empty(self.count, torch.float64) for _ in range(4 if with_mass else 3)]
@@ -1097,15 +1098,17 @@ class Simulation:
 
 # ----------------------------------------------------------------- adjoint
 class Adjoint:
-    """Reverse mode of `steps` KDK steps of the FP64 Barnes-Hut pipeline with
-    checkpoints every `checkpoint_every` steps (device tensors in and out)."""
+    """Reverse mode of `steps` KDK steps of the Barnes-Hut pipeline with
+    checkpoints every `checkpoint_every` steps (device tensors in and out).
+    precision PRECISION_FP64 differentiates the FP64 pipeline,
+    PRECISION_MIXED the mixed one (mixed forward steps, mixed force VJPs)."""
 
     def __init__(self, count: int, time_step: float, theta: float, softening: float,
                  steps: int, checkpoint_every: int, gravitational_constant: float = DEFAULT_G,
-                 bits_per_axis=21, stream=None):
+                 bits_per_axis=21, stream=None, precision=PRECISION_FP64):
         device()
         self.config = SimConfig(time_step, theta, softening, gravitational_constant,
-                                PRECISION_FP64, bits_per_axis)
+                                precision, bits_per_axis)
         self.count = count
         self.handle = _p()
         check(lib.nbh_adjoint_create(C.byref(self.handle), C.byref(self.config), count,

@@ -574,24 +574,63 @@ def small_plummer_end_to_end():
 
 
 # ------------------------------------------------------------- adjoint
-def adjoint_workload(args, probes=None, quiet=False):
-    """The differentiable workload: K leapfrog steps of the FP64 pipeline on
-    the disk + bulge galaxy, reverse mode with checkpoints, and the gradient
-    of L = sum_i |r_i(T) - r*_i|^2 (r* a perturbed copy of the final
-    positions) with respect to the initial velocities and the masses,
-    checked by central differences on a few coordinates."""
+# FLOPs of one interaction of the force VJP: separation and r^2 (9), rsq
+# and its powers (4), d . abar (5), the two coefficients (4), the three
+# gradient components (9), the target sum (3), the mass term (1)
+VJP_FLOPS_PER_INTERACTION = 35
+
+
+def vjp_roofline(nb, torch, p, precision, n_reps=3):
+    """Time of one force VJP on the disk + bulge state (walk, push-down and
+    combination; HIP events on the default stream, on which the library
+    runs) and its achieved FLOP rate over the interactions of that tree."""
+    s = nb.Simulation(p, 1e-3, 0.5, 0.01, gravitational_constant=1.0, precision=precision,
+                      bits_per_axis=21)
+    s.initialize()
+    pp, pn = s.count_interactions()
+    ab = [torch.from_numpy(np.random.default_rng(5).normal(size=len(p["m"]))) for _ in range(3)]
+    s.force_vjp(*ab)  # warm-up
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms = []
+    for _ in range(n_reps):
+        ev0.record()
+        s.force_vjp(*ab)
+        ev1.record()
+        torch.cuda.synchronize()
+        ms.append(ev0.elapsed_time(ev1))
+    s.close()
+    t = float(np.median(ms))
+    peak = FP32_VALU_PEAK_TFLOPS if precision == nb.PRECISION_MIXED else FP64_VALU_PEAK_TFLOPS
+    achieved = VJP_FLOPS_PER_INTERACTION * (pp + pn) / (t * 1e-3) / 1e12
+    return {"vjp_ms": round(t, 3), "interactions": pp + pn, "bound": "valu",
+            "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4),
+            "flops_per_interaction": VJP_FLOPS_PER_INTERACTION}
+
+
+def adjoint_workload(args, probes=None, quiet=False, precision="fp64"):
+    """The differentiable workload: K leapfrog steps on the disk + bulge
+    galaxy, reverse mode with checkpoints, and the gradient of
+    L = sum_i |r_i(T) - r*_i|^2 (r* a perturbed copy of the final positions)
+    with respect to the initial velocities and the masses.  precision
+    "fp64" differentiates the FP64 pipeline, "mixed" the mixed one.  Central
+    differences on a few coordinates of the FP64 evolution check either
+    gradient; the mixed gradient is also compared with the FP64 one."""
     import torch
     import nbody_amd as nb
 
     n, K, every = args.adjoint_n, args.adjoint_steps, args.adjoint_every
     nprobe = args.adjoint_probes if probes is None else probes
+    prec = nb.PRECISION_MIXED if precision == "mixed" else nb.PRECISION_FP64
     p = nb.generate("disk", n, 42)
-    adj = nb.Adjoint(n, 1e-3, 0.5, 0.01, K, every, gravitational_constant=1.0)
+    adj = nb.Adjoint(n, 1e-3, 0.5, 0.01, K, every, gravitational_constant=1.0, precision=prec)
+    ref = adj if prec == nb.PRECISION_FP64 else nb.Adjoint(
+        n, 1e-3, 0.5, 0.01, K, every, gravitational_constant=1.0, precision=nb.PRECISION_FP64)
     state = [torch.from_numpy(p[k]).cuda() for k in ("x", "y", "z", "vx", "vy", "vz")]
     m = torch.from_numpy(p["m"]).cuda()
 
     def final(mm, st):
-        return np.stack([t.cpu().numpy() for t in adj.forward(mm, *st)[:3]])
+        return np.stack([t.cpu().numpy() for t in ref.forward(mm, *st)[:3]])
 
     r_end = final(m, state)
     target = r_end + 1e-3 * np.random.default_rng(1).normal(size=r_end.shape)
@@ -601,6 +640,14 @@ def adjoint_workload(args, probes=None, quiet=False):
         # positions: sum d (d + 2 (r_end - r*)), d = r(T) - r_end
         d = final(mm, st) - r_end
         return float((d * (d + 2.0 * (r_end - target))).sum())
+
+    def gradient(a):
+        out = a.forward(m, *state)
+        cot = [torch.from_numpy(2.0 * (o.cpu().numpy() - target[k])).cuda()
+               for k, o in enumerate(out[:3])]
+        zeros = [torch.zeros_like(state[0]) for _ in range(3)]
+        g = a.backward(*cot, *zeros, with_mass=True)
+        return np.stack([t.cpu().numpy() for t in g[3:6]]), g[6].cpu().numpy()
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -650,19 +697,29 @@ def adjoint_workload(args, probes=None, quiet=False):
         "metric": "adjoint Mparticle-steps/s (forward + reverse), disk + bulge",
         "value": round(n * K / ((t1 - t0) + (t3 - t2)) / 1e6, 3),
         "unit": "Mparticle-steps/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f64", "data": "synthetic (seeded generator)",
+        "dtype": "f32 interactions / f64 accumulation" if prec == nb.PRECISION_MIXED else "f64",
+        "data": "synthetic (seeded generator)",
         "config": {"workload": f"disk+bulge N={n} theta=0.5 eps=0.01 dt=1e-3 K={K} "
                                f"checkpoints every {every}",
-                   "n_particles": n, "steps": K, "checkpoint_every": every},
+                   "n_particles": n, "steps": K, "checkpoint_every": every,
+                   "precision": precision},
         "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
         "backward_over_forward": round(bwd_ms / fwd_ms, 3),
         "loss": "sum |r(T) - r*|^2, r* = r(T) + 1e-3 N(0,1)",
         "fd_probes": results,
         # probes whose steps h and 2h agree to 1e-4 (no opening decision
-        # flipped within 2h), and their largest relative error
+        # flipped within 2h), and their largest relative error; the
+        # differences are of the FP64 evolution
         "fd_velocity_smooth_max_rel_err": worst(False),
         "fd_mass_smooth_max_rel_err": worst(True),
+        "vjp": vjp_roofline(nb, torch, p, prec),
     }
+    if ref is not adj:
+        gv64, gm64 = gradient(ref)
+        line["vs_fp64_gradient"] = {
+            "v0_rel_l2": float(np.linalg.norm(gv - gv64) / np.linalg.norm(gv64)),
+            "m_rel_l2": float(np.linalg.norm(gm - gm64) / np.linalg.norm(gm64))}
+        ref.close()
     adj.close()
     if not quiet:
         print(json.dumps(line))
@@ -700,7 +757,8 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="measure the headline workload only (no other configs)")
     ap.add_argument("--adjoint", action="store_true",
-                    help="measure only the differentiable workload (FP64 forward + reverse)")
+                    help="measure only the differentiable workload (forward + reverse)")
+    ap.add_argument("--adjoint-precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--adjoint-n", type=int, default=262_144)
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--adjoint-every", type=int, default=10)
@@ -712,7 +770,7 @@ def main():
         sys.exit(2)
 
     if args.adjoint:
-        adjoint_workload(args)
+        adjoint_workload(args, precision=args.adjoint_precision)
         return
 
     import torch
@@ -738,11 +796,15 @@ def main():
         cfg["reference_blob_100k"] = config_reference_blob(nb, torch, 100_000)
         cfg["reference_blob_250k"] = config_reference_blob(nb, torch, 250_000)
         log("reference blobs done")
+        keys = ("value", "unit", "dtype", "config", "forward_ms", "backward_ms",
+                "backward_over_forward", "fd_probes", "fd_velocity_smooth_max_rel_err",
+                "fd_mass_smooth_max_rel_err", "vjp", "vs_fp64_gradient")
         adj = adjoint_workload(args, probes=4, quiet=True)
-        log("adjoint done")
-        cfg["disk_bulge_adjoint"] = {k: adj[k] for k in (
-            "value", "unit", "config", "forward_ms", "backward_ms", "backward_over_forward",
-            "fd_probes", "fd_velocity_smooth_max_rel_err", "fd_mass_smooth_max_rel_err")}
+        cfg["disk_bulge_adjoint"] = {k: adj[k] for k in keys if k in adj}
+        log("adjoint FP64 done")
+        adj = adjoint_workload(args, probes=4, quiet=True, precision="mixed")
+        cfg["disk_bulge_adjoint_mixed"] = {k: adj[k] for k in keys if k in adj}
+        log("adjoint mixed done")
         line["configs"] = cfg
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(p, args)
     log("done")

@@ -341,18 +341,22 @@ int nbh_sim_count_interactions(nbh_sim* sim, uint64_t* pp, uint64_t* pn,
 int nbh_sim_energy(nbh_sim* sim, double* kinetic, double* potential);
 int nbh_sim_potential(nbh_sim* sim, double* d_phi);
 
-/* Force vector-Jacobian product at the current tree of an FP64 simulation:
- * g = (da/dx)^T abar and, when d_gm != NULL, gm = (da/dm)^T abar, the tree
- * (interaction lists) held fixed.  A source node's mass and centre of mass
- * are differentiated through to its particles.  All arrays are device arrays
- * of count in original order.  Pairs with nbh_sim_accelerations as the
- * forward. */
+/* Force vector-Jacobian product at the current tree: g = (da/dx)^T abar
+ * and, when d_gm != NULL, gm = (da/dm)^T abar, the tree (interaction lists)
+ * held fixed.  A source node's mass and centre of mass are differentiated
+ * through to its particles.  An FP64 simulation differentiates its FP64
+ * forces; a mixed one its mixed forces, along the mixed walk (FP32
+ * interactions on fixed-point separations, compensated target sums, FP64
+ * source and node accumulation and push-down).  All arrays are device
+ * arrays of count in original order.  Pairs with nbh_sim_accelerations as
+ * the forward. */
 int nbh_sim_force_vjp(nbh_sim* sim, const double* d_abx, const double* d_aby,
                       const double* d_abz, double* d_gx, double* d_gy,
                       double* d_gz, double* d_gm);
 
-/* Reverse mode of `steps` kick-drift-kick steps (FP64 pipeline, tree fixed
- * within each force evaluation).  Forward takes the initial state (device
+/* Reverse mode of `steps` kick-drift-kick steps (the FP64 or the mixed
+ * pipeline, as config->precision says; tree fixed within each force
+ * evaluation).  Forward takes the initial state (device
  * arrays, original order), stores (x, v) every checkpoint_every steps and
  * returns the final state; backward takes the cotangents of the final
  * positions and velocities and returns those of the initial positions,
