@@ -20,7 +20,7 @@ namespace nbh {
 namespace {
 
 constexpr int kStaticLeafGroupParticles = NBH_LEAF_GROUPS_NOALLOC_MAX;
-constexpr size_t kStaticScanTile = 8192; // the scan's keys per tile (checked per call)
+constexpr size_t kStaticScanTile = 2048; // the scan's smallest tile (checked per call)
 constexpr size_t kStaticStatusWords =
     (static_cast<size_t>(kStaticLeafGroupParticles) + kStaticScanTile - 1) / kStaticScanTile + 64;
 // status words, then one word whose low half holds the group count

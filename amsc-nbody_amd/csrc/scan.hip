@@ -21,17 +21,28 @@
 #include "device_utils.hpp"
 #include "nbh_internal.hpp"
 
+#include <type_traits>
+
 namespace nbh {
 
 namespace {
 
-// 8192-element tiles: the inclusive prefix travels back from the first
-// tile by about 64 tiles per look-back round trip, so the tile count bounds
-// the scan's latency (2048-element tiles: about 2x slower at 8M elements).
-constexpr int kScanThreads = 1024;
-constexpr int kScanWaves = kScanThreads / 64;
+// Tiles of THREADS x 8 elements.  The inclusive prefix travels back from the
+// first tile by about 64 tiles per look-back round trip, so the tile count
+// bounds the scan's latency: large scans take 8192-element tiles (1024
+// threads; 2048-element tiles are about 2x slower at 8M elements), small
+// ones 2048-element tiles, which keep more of the device busy.
 constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kScanThreadsLarge = 1024;
+constexpr int kScanThreadsSmall = 256;
+constexpr int kScanTileSmall = kScanThreadsSmall * kScanItems;
+constexpr size_t kScanLargeFrom = 2u << 20; // elements
+
+template <int THREADS>
+constexpr int scan_tile()
+{
+    return THREADS * kScanItems;
+}
 constexpr unsigned long long kAgg = 1ull << 62;
 constexpr unsigned long long kInc = 2ull << 62;
 constexpr unsigned long long kPayload = (1ull << 62) - 1ull;
@@ -88,10 +99,11 @@ __device__ __forceinline__ void unpack(unsigned long long w, long long& v)
 // On return v[] holds the global inclusive values.  `identity` must be the
 // op's identity, packed as payload 0.  The first wave performs the
 // look-back.
-template <typename T>
+template <int THREADS, typename T>
 __device__ void tile_scan(T (&v)[kScanItems], T identity, uint32_t tile,
                           unsigned long long* status)
 {
+    constexpr int kScanWaves = THREADS / 64;
     __shared__ T s_wave[kScanWaves];
     __shared__ T s_prefix;
     const int lane = threadIdx.x & 63;
@@ -174,8 +186,8 @@ __device__ void tile_scan(T (&v)[kScanItems], T identity, uint32_t tile,
         v[k] = combine(before, v[k]);
 }
 
-template <typename K>
-__global__ __launch_bounds__(kScanThreads) void leaf_groups_kernel(
+template <typename K, int THREADS>
+__global__ __launch_bounds__(THREADS) void leaf_groups_kernel(
     const K* __restrict__ keys, int n, K* __restrict__ unique,
     int* __restrict__ first, int* __restrict__ count, int* __restrict__ ngroups,
     unsigned long long* status, unsigned* ticket)
@@ -185,7 +197,7 @@ __global__ __launch_bounds__(kScanThreads) void leaf_groups_kernel(
         s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
-    const int base = static_cast<int>(tile) * kScanTile + threadIdx.x * kScanItems;
+    const int base = static_cast<int>(tile) * scan_tile<THREADS>() + threadIdx.x * kScanItems;
     K k[kScanItems + 2];
     // k[0] = key[base-1], k[1..ITEMS] = keys, k[ITEMS+1] = key[base+ITEMS]
 #pragma unroll
@@ -200,7 +212,7 @@ __global__ __launch_bounds__(kScanThreads) void leaf_groups_kernel(
         const bool head = i < n && (i == 0 || k[j + 1] != k[j]);
         v[j] = {head ? 1 : 0, head ? i : 0};
     }
-    tile_scan<SumMax>(v, SumMax{0, 0}, tile, status);
+    tile_scan<THREADS, SumMax>(v, SumMax{0, 0}, tile, status);
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const int i = base + j;
@@ -224,8 +236,8 @@ __global__ __launch_bounds__(kScanThreads) void leaf_groups_kernel(
 
 // radix_to_octree.cu:54-68 (spatial prefix, level) and :70-145 (per-node
 // plan with error codes), fused with the exclusive scan of edge counts.
-template <int BITS>
-__global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(
+template <int BITS, int THREADS>
+__global__ __launch_bounds__(THREADS) void plan_scan_kernel(
     const int* __restrict__ d_nleaves, const int* __restrict__ tparent,
     const int* __restrict__ prefix_len,
     const typename Layout<BITS>::Key* __restrict__ unique,
@@ -245,9 +257,9 @@ __global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(
     const int nl = *d_nleaves;
     const int R = 2 * nl - 1;
     const uint32_t tile = s_tile;
-    if (static_cast<long long>(tile) * kScanTile >= R)
+    if (static_cast<long long>(tile) * scan_tile<THREADS>() >= R)
         return; // beyond the last tile: nobody looks back at it
-    const int base = static_cast<int>(tile) * kScanTile + threadIdx.x * kScanItems;
+    const int base = static_cast<int>(tile) * scan_tile<THREADS>() + threadIdx.x * kScanItems;
     auto level_of = [&](int node) {
         const int sp = (node < nl) ? L::spatial_bits
                                    : prefix_len[node - nl] - L::padding_bits;
@@ -320,7 +332,7 @@ __global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(
             }
         }
     }
-    tile_scan<long long>(v, 0ll, tile, status);
+    tile_scan<THREADS, long long>(v, 0ll, tile, status);
     // tile_scan's barriers order the LDS counts before these reads
     if (level_count && threadIdx.x < kMaxLevels && s_lc[threadIdx.x] != 0)
         atomicAdd(&level_count[threadIdx.x * kLevelCountStride], s_lc[threadIdx.x]);
@@ -344,7 +356,7 @@ __global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(
 
 size_t scan_status_words(size_t n)
 {
-    return (n + kScanTile - 1) / kScanTile + 64;
+    return (n + kScanTileSmall - 1) / kScanTileSmall + 64;
 }
 
 // status: scan_status_words(n) zeroed 64-bit words; the ticket lives in the
@@ -354,10 +366,17 @@ void launch_leaf_groups(hipStream_t s, const K* keys, int n, K* unique,
                         int* first, int* count, int* d_ngroups,
                         unsigned long long* status)
 {
-    const size_t tiles = (static_cast<size_t>(n) + kScanTile - 1) / kScanTile;
-    unsigned* ticket = reinterpret_cast<unsigned*>(status + tiles + 32);
-    leaf_groups_kernel<K><<<static_cast<unsigned>(tiles), kScanThreads, 0, s>>>(
-        keys, n, unique, first, count, d_ngroups, status, ticket);
+    const auto go = [&](auto threads) {
+        constexpr int T = decltype(threads)::value;
+        const size_t tiles = (static_cast<size_t>(n) + scan_tile<T>() - 1) / scan_tile<T>();
+        unsigned* ticket = reinterpret_cast<unsigned*>(status + tiles + 32);
+        leaf_groups_kernel<K, T><<<static_cast<unsigned>(tiles), T, 0, s>>>(
+            keys, n, unique, first, count, d_ngroups, status, ticket);
+    };
+    if (static_cast<size_t>(n) >= kScanLargeFrom)
+        go(std::integral_constant<int, kScanThreadsLarge>{});
+    else
+        go(std::integral_constant<int, kScanThreadsSmall>{});
     NBH_HIP(hipGetLastError());
 }
 
@@ -371,11 +390,18 @@ void launch_plan_scan(hipStream_t s, const int* d_nleaves, int max_leaves,
                       const int* parent_plen)
 {
     const size_t rmax = 2 * static_cast<size_t>(max_leaves);
-    const size_t tiles = (rmax + kScanTile - 1) / kScanTile;
-    unsigned* ticket = reinterpret_cast<unsigned*>(status + tiles + 32);
-    plan_scan_kernel<BITS><<<static_cast<unsigned>(tiles), kScanThreads, 0, s>>>(
-        d_nleaves, tparent, prefix_len, unique, level, count, offset,
-        d_noctree, err, status, ticket, level_count, parent_plen);
+    const auto go = [&](auto threads) {
+        constexpr int T = decltype(threads)::value;
+        const size_t tiles = (rmax + scan_tile<T>() - 1) / scan_tile<T>();
+        unsigned* ticket = reinterpret_cast<unsigned*>(status + tiles + 32);
+        plan_scan_kernel<BITS, T><<<static_cast<unsigned>(tiles), T, 0, s>>>(
+            d_nleaves, tparent, prefix_len, unique, level, count, offset, d_noctree, err,
+            status, ticket, level_count, parent_plen);
+    };
+    if (rmax >= kScanLargeFrom)
+        go(std::integral_constant<int, kScanThreadsLarge>{});
+    else
+        go(std::integral_constant<int, kScanThreadsSmall>{});
     NBH_HIP(hipGetLastError());
 }
 

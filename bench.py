@@ -759,6 +759,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="measure only the differentiable workload (forward + reverse)")
     ap.add_argument("--adjoint-precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--config", choices=("uniform_1m", "clumps_16m", "blob_100k", "blob_250k"),
+                    help="measure only this configuration and print its JSON object")
     ap.add_argument("--adjoint-n", type=int, default=262_144)
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--adjoint-every", type=int, default=10)
@@ -775,6 +777,14 @@ def main():
 
     import torch
     import nbody_amd as nb
+
+    if args.config:
+        legs = {"uniform_1m": lambda: config_uniform_1m(nb, torch, args),
+                "clumps_16m": lambda: config_clumps_16m(nb, torch, args),
+                "blob_100k": lambda: config_reference_blob(nb, torch, 100_000),
+                "blob_250k": lambda: config_reference_blob(nb, torch, 250_000)}
+        print(json.dumps({args.config: legs[args.config]()}))
+        return
 
     if args.calibrate_copy_mib > 0:
         words = args.calibrate_copy_mib * (1 << 20) // 8

@@ -164,3 +164,51 @@ def test_mixed_adjoint_matches_fp64_finite_differences(nb):
           f"rel error {err:.2e}")
     assert ok.sum() >= 56
     assert err < TOL_FD
+
+
+def test_mixed_adjoint_full_size_velocity_probes(nb):
+    # the differentiable workload at its own size: disk + bulge, N = 262,144,
+    # K = 100, checkpoints every 10; the mixed gradient of
+    # L = sum (r(T) - r*)^2 with respect to 8 initial velocity components
+    # against central differences of the FP64 evolution
+    n, K, every = 262_144, 100, 10
+    p = nb.generate("disk", n, 42)
+    state = [torch.from_numpy(p[k]).cuda() for k in ("x", "y", "z", "vx", "vy", "vz")]
+    m = torch.from_numpy(p["m"]).cuda()
+    ref = _adjoint(nb, n, K, every, nb.PRECISION_FP64)
+
+    def final(st):
+        return np.stack([t.cpu().numpy() for t in ref.forward(m, *st)[:3]])
+
+    r_end = final(state)
+    target = r_end + 1e-3 * np.random.default_rng(1).normal(size=r_end.shape)
+
+    def loss(st):
+        d = final(st) - r_end
+        return float((d * (d + 2.0 * (r_end - target))).sum())
+
+    adj = _adjoint(nb, n, K, every, nb.PRECISION_MIXED)
+    out = adj.forward(m, *state)
+    cot = [torch.from_numpy(2.0 * (o.cpu().numpy() - target[k])).cuda()
+           for k, o in enumerate(out[:3])]
+    zeros = [torch.zeros_like(state[0]) for _ in range(3)]
+    g = adj.backward(*cot, *zeros)
+    gv = np.stack([t.cpu().numpy() for t in g[3:6]])
+    rng = np.random.default_rng(2)
+    errs, smooth = [], 0
+    for _ in range(8):
+        i, c, h = int(rng.integers(n)), int(rng.integers(3)), 1e-7
+
+        def f(d):
+            st = [t.clone() for t in state]
+            st[3 + c][i] += d
+            return loss(st)
+        d1 = (f(h) - f(-h)) / (2 * h)
+        d2 = (f(2 * h) - f(-2 * h)) / (4 * h)
+        if abs(d1 - d2) <= 1e-4 * abs(d1):
+            smooth += 1
+            errs.append(abs(gv[c, i] - d1) / abs(d1))
+    print(f"mixed adjoint (disk N={n}, K={K}) vs FP64 FD on 8 velocity components: "
+          f"{smooth} smooth, max rel error {max(errs) if errs else float('nan'):.2e}")
+    assert smooth >= 4
+    assert max(errs) < TOL_FD
