@@ -96,9 +96,19 @@ __global__ void queue_clear_kernel(int* __restrict__ queue)
 
 } // namespace
 
-// Blocks of 4 persistent wavefronts: one wavefront per resident slot, 32
-// per CU (8 per SIMD), fewer when there are fewer pairs of units.
-inline int walk_persistent_blocks(long long units)
+// Blocks of 4 persistent wavefronts: one wavefront per unit_per_wave units
+// of work, at most waves_per_simd per SIMD (4 SIMDs per CU).  The cap is
+// what is resident at once, so that no wavefront starts after others have
+// emptied the queue.  gfx950 keeps 8 wavefronts per SIMD resident only for
+// kernels of at most 64 VGPRs and 80 SGPRs (VCC included): kernels of 81,
+// 85 and 89 SGPRs were measured at 7, of 97 SGPRs at 6.
+// Resident wavefronts per SIMD of the whole-walk traversal kernels at their
+// register counts (tests/test_kernel_resources.py checks the counts).
+constexpr int kWalkMixedWavesPerSimd = 8;
+constexpr int kWalkFp64WavesPerSimd = 7;      // reduced arithmetic, no containment test
+constexpr int kWalkFp64WavesPerSimdOther = 6; // the other FP64 variants
+
+inline int walk_persistent_blocks(long long units, int units_per_wave, int waves_per_simd)
 {
     static int cus[64] = {};
     int dev = 0;
@@ -106,11 +116,11 @@ inline int walk_persistent_blocks(long long units)
     int& c = cus[dev & 63];
     if (c == 0)
         NBH_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
-    const long long pairs = (units + 1) / 2;
-    const long long cap = 32LL * c;
-    const int waves = static_cast<int>(pairs < cap ? pairs : cap);
+    const long long want = (units + units_per_wave - 1) / units_per_wave;
+    const long long cap = 4LL * waves_per_simd * c;
+    const int waves = static_cast<int>(want < cap ? want : cap);
     return (waves + 3) / 4;
-} // 4 independent wavefronts per block
+}
 
 // Static grids (the FP64 walks): packets are Morton-consecutive; blocks are
 // dealt to the 8 XCDs round

@@ -456,7 +456,11 @@ void launch_walk_fp64(hipStream_t s, const Walk64Rec* w64, size_t wcap,
     queue_clear_kernel<<<1, 64, 0, s>>>(queue);
     NBH_HIP(hipGetLastError());
     const long long packets = (static_cast<long long>(n) + 63) / 64;
-    const int nb = walk_persistent_blocks(packets);
+    // one pair of packets per wavefront, as many as stay resident: 7 per
+    // SIMD for the reduced arithmetic without the containment test, 6 for
+    // the other variants (more registers)
+    const int nb = walk_persistent_blocks(
+        packets, 2, (fast && !contains) ? kWalkFp64WavesPerSimd : kWalkFp64WavesPerSimdOther);
     if (nb == 0)
         return;
 #define NBH_ARGS                                                                 \

@@ -484,7 +484,7 @@ __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__
 
 // A whole walk writes the target's acceleration; a split unit (part given)
 // writes its FP64 partial sums at part[(unit * 64 + lane) * 3].
-template <bool COUNT>
+template <bool COUNT, bool SPLIT>
 __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ idx,
                                        int n, double out_scale,
                                        double* __restrict__ ax_out,
@@ -499,7 +499,7 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
     }
     if (COUNT || !w.valid)
         return;
-    if (part) {
+    if (SPLIT) {
         double* q = part + (static_cast<size_t>(w.unit) * 64 + (threadIdx.x & 63)) * 3;
         q[0] = w.a.kx.value();
         q[1] = w.a.ky.value();
@@ -558,11 +558,10 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             w.i = first;
             w.next = w.valid ? first : kIdle;
             w.end = split[2 + 4 * k];
+            w.unit = u;
         } else {
             w.start(rec, order, u, lane, n);
-            w.end = wcount;
         }
-        w.unit = u;
     };
     Walker A, B;
     A.npp = A.npn = A.visits = 0;
@@ -619,13 +618,13 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
                 ++A.visits;
             visit_boundary<CONTAINS, SOFT, COUNT>(A, ra, split, wcenter, rec, eps2);
             end_visit(A);
-        } else if (A.i < A.end) {
+        } else if (A.i < (SPLIT ? A.end : wcount)) {
             if (COUNT)
                 ++A.visits;
             visit_rare<SOFT, COUNT>(A, ra, rec, eps2);
             end_visit(A);
         } else {
-            finish<COUNT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+            finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
             const int u = q.take();
             if (u >= 0)
                 begin(A, u);
@@ -642,13 +641,13 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
                 ++B.visits;
             visit_boundary<CONTAINS, SOFT, COUNT>(B, rb, split, wcenter, rec, eps2);
             end_visit(B);
-        } else if (B.i < B.end) {
+        } else if (B.i < (SPLIT ? B.end : wcount)) {
             if (COUNT)
                 ++B.visits;
             visit_rare<SOFT, COUNT>(B, rb, rec, eps2);
             end_visit(B);
         } else {
-            finish<COUNT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+            finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
             const int u = q.take();
             if (u >= 0)
                 begin(B, u);
@@ -658,11 +657,11 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     }
     if (run_a) {
         walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(A, wrec, wcenter, rec, split, wcount, eps2);
-        finish<COUNT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+        finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
     }
     if (run_b) {
         walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(B, wrec, wcenter, rec, split, wcount, eps2);
-        finish<COUNT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+        finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
     }
     if (COUNT) {
         unsigned long long npp = A.npp + B.npp, npn = A.npn + B.npn;
@@ -805,7 +804,13 @@ void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
         NBH_HIP(hipGetLastError());
     }
     const long long packets = (static_cast<long long>(n) + 63) / 64;
-    const int nb = walk_persistent_blocks(splits ? packets * kWalkSplitMax : packets);
+    // Whole walks: one pair of packets per wavefront, 8 resident per SIMD.
+    // Split walks: about eight subtree units per wavefront; their short
+    // units run faster with fewer wavefronts (100K particles: 1.25 ms per
+    // step at 2 per SIMD against 1.40 ms at 8; 250K: 1.63 ms at 4 against
+    // 1.82 ms at 8).
+    const int nb = splits ? walk_persistent_blocks(packets * kWalkSplitMax, 8, 8)
+                          : walk_persistent_blocks(packets, 2, kWalkMixedWavesPerSimd);
     if (nb == 0)
         return;
 #define NBH_ARGS                                                                 \
