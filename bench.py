@@ -74,6 +74,22 @@ def momentum(s):
     return np.array([math.fsum(s["m"] * s[k]) for k in ("vx", "vy", "vz")])
 
 
+# Simulations no longer needed are closed at the end of the run, not
+# between measurements: freeing gigabytes of device memory just before a
+# timed region (on a device whose memory a previous process fragmented) has
+# stalled a later device synchronisation inside the timed region by 30-50 ms.
+_RETIRED = []
+
+
+def retire(sim):
+    _RETIRED.append(sim)
+
+
+def release_retired():
+    while _RETIRED:
+        _RETIRED.pop().close()
+
+
 def time_steps(torch, sim, steps):
     """Seconds for `steps` steps with the per-phase timing events off."""
     sim.set_timing(False)
@@ -149,7 +165,7 @@ def energy_of(nb, state, dt, theta, eps, G):
                       precision=nb.PRECISION_FP64, bits_per_axis=21)
     s.initialize()
     k, u = s.energy()
-    s.close()
+    retire(s)
     return k + u, k, u
 
 
@@ -178,7 +194,7 @@ def drift_run(nb, torch, p, theta, args, steps, targets=None, exact=None):
     f64 = nb.Simulation(p, args.dt, theta, args.softening, precision=nb.PRECISION_FP64, **common)
     f64.initialize()
     a64 = f64.accelerations()
-    f64.close()
+    retire(f64)
     out = {}
     out["rms_vs_fp64_bh"], out["max_vs_fp64_bh"] = map(g4, rel_err(a_mixed, a64))
     if exact is not None:
@@ -191,7 +207,7 @@ def drift_run(nb, torch, p, theta, args, steps, targets=None, exact=None):
     secs = time_steps(torch, mixed, steps)
     s1 = mixed.state()
     e1 = energy_of(nb, s1, args.dt, theta, args.softening, args.g)
-    mixed.close()
+    retire(mixed)
     out.update({"steps": steps, "ms_per_step": round(1e3 * secs / steps, 3),
                 "energy_rel_drift": g4((e1[0] - e0[0]) / abs(e0[0])),
                 "momentum_drift": g4(float(np.abs(momentum(s1) - momentum(s0)).max()))})
@@ -218,7 +234,11 @@ def headline(nb, torch, args):
     s0 = mixed.state()
     e0 = None if args.no_energy else energy_of(nb, s0, args.dt, args.theta, args.softening, args.g)
     for _ in range(args.warmup):
+        # untimed; each followed by a device synchronisation, so that a
+        # one-off stall of the runtime lands here rather than in the timed
+        # region
         mixed.step()
+        torch.cuda.synchronize()
     secs = time_steps(torch, mixed, args.steps)
     phases = phase_steps(mixed)
     # further steps to the drift span (50 steps in all by default)
@@ -244,7 +264,7 @@ def headline(nb, torch, args):
         f64.step()  # warm-up
         secs64 = time_steps(torch, f64, args.fp64_steps)
         phases64 = phase_steps(f64, 1)
-        f64.close()
+        retire(f64)
         fp64 = {"value": round(n * args.fp64_steps / secs64 / 1e6, 3),
                 "ms_per_step": round(1e3 * secs64 / args.fp64_steps, 3),
                 "steps": args.fp64_steps, "phase_ms": phases64}
@@ -267,7 +287,7 @@ def headline(nb, torch, args):
                                           exact)
 
     frames = frame_capture_timing(nb, torch, mixed, n)
-    mixed.close()
+    retire(mixed)
 
     trav_ms = phases["traversal"]
     achieved = FLOPS_PER_INTERACTION * interactions / (trav_ms * 1e-3) / 1e12
@@ -358,7 +378,7 @@ def config_uniform_1m(nb, torch, args):
         leg.update({"steps": steps, "ms_per_step": round(1e3 * secs / steps, 3),
                     "Mparticle_steps_per_s": round(n * steps / secs / 1e6, 3)})
         leg["phase_ms"] = phase_steps(sim, 1)
-        sim.close()
+        retire(sim)
         out[name] = leg
     out["mixed_over_fp64"] = round(out["mixed"]["Mparticle_steps_per_s"] /
                                    out["fp64"]["Mparticle_steps_per_s"], 3)
@@ -387,12 +407,12 @@ def config_clumps_16m(nb, torch, args):
     s1 = mixed.state()
     e1 = energy_of(nb, s1, dt, theta, eps, 1.0)
     g, m = sizes(mixed)
-    mixed.close()
+    retire(mixed)
     f64 = nb.Simulation(p, dt, theta, eps, precision=nb.PRECISION_FP64, **common)
     f64.initialize()
     a64 = f64.accelerations()
     secs64 = time_steps(torch, f64, 3)
-    f64.close()
+    retire(f64)
     trav = phases["traversal"]
     tf = FLOPS_PER_INTERACTION * (pp + pn) / (trav * 1e-3) / 1e12
     r_bh, mx_bh = rel_err(a_mixed, a64)
@@ -431,7 +451,7 @@ def config_reference_blob(nb, torch, n):
         out[f"{name}_ms_per_step"] = round(1e3 * secs / 10, 3)
         if prec == nb.PRECISION_MIXED:
             out["mixed_phase_ms"] = phase_steps(sim, 2)
-        sim.close()
+        retire(sim)
     ref = {100_000: 249.0, 250_000: 798.0}.get(n)
     if ref:
         out["reference_fp64_ms_per_step_rtx3070"] = ref
@@ -569,7 +589,7 @@ def small_plummer_end_to_end():
         else:
             dx = np.sqrt(sum((st[f] - rs[f]) ** 2 for f in ("x", "y", "z")))
             out["mixed_max_position_diff"] = float(dx.max())
-        sim.close()
+        retire(sim)
     return out
 
 
@@ -598,7 +618,7 @@ def vjp_roofline(nb, torch, p, precision, n_reps=3):
         ev1.record()
         torch.cuda.synchronize()
         ms.append(ev0.elapsed_time(ev1))
-    s.close()
+    retire(s)
     t = float(np.median(ms))
     peak = FP32_VALU_PEAK_TFLOPS if precision == nb.PRECISION_MIXED else FP64_VALU_PEAK_TFLOPS
     achieved = VJP_FLOPS_PER_INTERACTION * (pp + pn) / (t * 1e-3) / 1e12
@@ -719,8 +739,8 @@ def adjoint_workload(args, probes=None, quiet=False, precision="fp64"):
         line["vs_fp64_gradient"] = {
             "v0_rel_l2": float(np.linalg.norm(gv - gv64) / np.linalg.norm(gv64)),
             "m_rel_l2": float(np.linalg.norm(gm - gm64) / np.linalg.norm(gm64))}
-        ref.close()
-    adj.close()
+        retire(ref)
+    retire(adj)
     if not quiet:
         print(json.dumps(line))
     return line
@@ -730,7 +750,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4_194_304)
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--softening", type=float, default=1e-3)
@@ -816,6 +836,7 @@ def main():
         cfg["disk_bulge_adjoint_mixed"] = {k: adj[k] for k in keys if k in adj}
         log("adjoint mixed done")
         line["configs"] = cfg
+    release_retired()
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(p, args)
     log("done")
     print(json.dumps(line))
