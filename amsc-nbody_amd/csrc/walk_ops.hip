@@ -18,8 +18,14 @@
 // the octree, accumulating xbar_A / M_A and mbar_A - COM_A . xbar_A / M_A
 // from the root; each leaf particle adds m_k times the first sum to its
 // position cotangent, and the second sum plus x_k . (first sum) to its mass
-// cotangent.  Source cotangents are reduced over the wavefront (every lane
-// sees the same source) before one atomic add per component.
+// cotangent.  Source cotangents are summed over the wavefront (every lane
+// sees the same source) in batches: the lanes park their terms of a few
+// source visits in LDS, the batch is summed transposed (each lane one
+// component of one visit over a group of lanes, then across the group), and
+// one atomic instruction adds the totals to the FP64 accumulators.  The
+// record of each walk position is read into scalar registers, so that the
+// walk's branches stay uniform.
+#include "fp64_arith.hpp"
 #include "walk_common.hpp"
 
 namespace nbh {
@@ -46,11 +52,55 @@ struct OpArgs {
     double* nm;                    // node-source mass terms, by walk position
 };
 
-__device__ __forceinline__ double wave_sum(double v)
+
+// The walk position is wave-uniform; records are read into scalar registers
+// (one scalar load and its wait in one statement, so that no output register
+// is written after the statement ends), which keeps every record-driven
+// branch of the walk uniform.
+typedef int vjp_int8v __attribute__((ext_vector_type(8)));
+typedef int vjp_int16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ Walk64Rec load_record64(const Walk64Rec* w64, int i)
 {
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o);
-    return v;
+    const Walk64Rec* p = w64 + static_cast<size_t>(__builtin_amdgcn_readfirstlane(i));
+    vjp_int16v v;
+    asm volatile("s_load_dwordx16 %0, %1, 0x0\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=s"(v)
+                 : "s"(p)
+                 : "memory");
+    return __builtin_bit_cast(Walk64Rec, v);
+}
+
+// The {node, link} pair of the mixed records at position i.
+__device__ __forceinline__ void load_record32(const int4* wrec, int i, int4& nd, int4& lk)
+{
+    const int4* p = wrec + 2 * static_cast<size_t>(__builtin_amdgcn_readfirstlane(i));
+    vjp_int8v v;
+    asm volatile("s_load_dwordx8 %0, %1, 0x0\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=s"(v)
+                 : "s"(p)
+                 : "memory");
+    nd = make_int4(v[0], v[1], v[2], v[3]);
+    lk = make_int4(v[4], v[5], v[6], v[7]);
+}
+
+constexpr int kSrcBatch64 = 2;
+
+// 1 / sqrt(r2) for the potential and the force VJP, which are checked
+// against tolerances rather than bit for bit: the hardware estimate and two
+// Newton steps y (3 - r2 y^2) / 2 (relative error near 1e-16) inside
+// 2^-600 <= r2 <= 2^600, the full-precision library division outside.
+__device__ __forceinline__ double rsqrt_fp64(double r2)
+{
+    if (!in_fast_range(r2))
+        return 1.0 / sqrt(r2);
+    double y = __builtin_amdgcn_rsq(r2);
+    double h = 0.5 * r2;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
 }
 
 template <int OP>
@@ -82,14 +132,15 @@ struct Lane {
                 if (sm != 0.0)
                     code = 6;
             } else {
-                const double inv = 1.0 / sqrt(r2);
+                const double inv = rsqrt_fp64(r2);
                 const double gm = G * sm;
                 if (OP == OP_POTENTIAL) {
                     phi = fma(-gm, inv, phi);
                 } else {
-                    const double inv3 = inv * inv * inv;
+                    const double inv2 = inv * inv;
+                    const double inv3 = inv2 * inv;
                     const double dot = fma(dz, bz, fma(dx, bx, dy * by));
-                    const double c5 = 3.0 * dot * inv3 / r2;
+                    const double c5 = 3.0 * dot * inv3 * inv2;
                     cx = gm * fma(inv3, bx, -c5 * dx);
                     cy = gm * fma(inv3, by, -c5 * dy);
                     cz = gm * fma(inv3, bz, -c5 * dz);
@@ -101,19 +152,64 @@ struct Lane {
             }
         }
         if (OP == OP_VJP) {
-            if (__ballot(cx != 0.0 || cy != 0.0 || cz != 0.0 || cm != 0.0) != 0ull) {
-                cx = wave_sum(cx);
-                cy = wave_sum(cy);
-                cz = wave_sum(cz);
-                cm = wave_sum(cm);
-                if ((threadIdx.x & 63) == 0) {
-                    atomicAdd(dx_out + index, cx);
-                    atomicAdd(dy_out + index, cy);
-                    atomicAdd(dz_out + index, cz);
-                    atomicAdd(m_out + index, cm);
-                }
-            }
+            if (__ballot(cx != 0.0 || cy != 0.0 || cz != 0.0 || cm != 0.0) != 0ull)
+                park(cx, cy, cz, cm, dx_out, index);
         }
+    }
+
+    // ---- the wave's batch of source terms (OP_VJP; wave-uniform calls):
+    // the four terms of each lane for kSrcBatch64 source visits parked in
+    // LDS, then summed transposed, as in the mixed product
+    double4* slots = nullptr; // kSrcBatch64 x 64 lanes
+    int2* meta = nullptr;     // per parked visit: source index, node flag
+    int parked = 0;
+    double* pbase = nullptr;
+    double* nbase = nullptr;
+    size_t pstride = 0, nstride = 0;
+
+    __device__ __forceinline__ void park(double cx, double cy, double cz, double cm,
+                                         const double* dx_out, int index)
+    {
+        const int l = threadIdx.x & 63;
+        slots[parked * 64 + l] = make_double4(cx, cy, cz, cm);
+        if (l == 0)
+            meta[parked] = make_int2(index, dx_out == nbase ? 1 : 0);
+        if (++parked == kSrcBatch64)
+            flush_sources();
+    }
+
+    // Lane l: component (l >> 3) & 3 of parked visit l >> 5, summed over
+    // lanes (l & 7) * 8 .. + 7 of the visit, then over the group of eight.
+    __device__ __forceinline__ void flush_sources()
+    {
+        if (parked == 0)
+            return;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int l = threadIdx.x & 63;
+        const int v = l >> 5;
+        const int c = (l >> 3) & 3;
+        const int j = l & 7;
+        double sum = 0.0;
+        if (v < parked) {
+            const double* f = reinterpret_cast<const double*>(slots + v * 64 + j * 8) + c;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                sum += f[4 * k];
+        }
+        for (int o = 4; o > 0; o >>= 1)
+            sum += __shfl_xor(sum, o, 8);
+        if (j == 0 && v < parked) {
+            const int2 md = meta[v];
+            double* out = md.y ? nbase + static_cast<size_t>(c) * nstride
+                               : pbase + static_cast<size_t>(c) * pstride;
+            atomicAdd(out + md.x, sum);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        parked = 0;
     }
 };
 
@@ -133,6 +229,17 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
     const bool valid = t < n;
     const int wcount = *d_wcount;
     Lane<OP> L;
+    __shared__ double4 s_slots64[OP == OP_VJP ? kWalkThreads / 64 : 1]
+                                [OP == OP_VJP ? kSrcBatch64 * 64 : 1];
+    __shared__ int2 s_meta64[OP == OP_VJP ? kWalkThreads / 64 : 1][kSrcBatch64];
+    if (OP == OP_VJP) {
+        L.slots = s_slots64[threadIdx.x >> 6];
+        L.meta = s_meta64[threadIdx.x >> 6];
+        L.pbase = a.pbx;
+        L.nbase = a.nbx;
+        L.pstride = static_cast<size_t>(a.pby - a.pbx);
+        L.nstride = static_cast<size_t>(a.nby - a.nbx);
+    }
     uint32_t o = 0;
     if (valid) {
         const double4 p = rec[t];
@@ -153,7 +260,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
     int next = valid ? 0 : 0x7FFFFFFF;
     int i = 0;
     while (i < wcount) {
-        const Walk64Rec r = w64[i];
+        const Walk64Rec r = load_record64(w64, i);
         const int skip = r.skip;
         const int kind = r.kind;
         const bool active = next <= i && L.code == 0;
@@ -206,6 +313,8 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
         }
         i = (__ballot(open) != 0ull) ? i + 1 : skip;
     }
+    if (OP == OP_VJP)
+        L.flush_sources();
     if (L.code != 0) {
         raise_error(err, L.code);
         return;
@@ -351,6 +460,10 @@ __global__ __launch_bounds__(256) void energy_partials_kernel(
 // terms are summed per lane in FP32 partials folded into Kahan sums every
 // 64 visits; each source's terms are summed over the wave in FP32 and added
 // to FP64 accumulators (particles by Morton index, nodes by walk position).
+// The source sums are batched: a wave parks the four terms of each lane for
+// kSrcBatch source visits in LDS, then transposes them, so that one batch
+// costs a few adds per lane and one atomic instruction, where a sum per
+// visit took four six-step cross-lane reductions and four atomics.
 struct MixedVjp {
     const int4* wrec;    // interleaved {node, link} pairs, 2 int4 per position
     const int4* wcenter; // cell centres (theta >= 1.15), else null
@@ -364,30 +477,19 @@ struct MixedVjp {
     float eps2;          // softening^2 in fixed-point units
     double gunit;        // G m_max unit^3
     double munit;        // G unit^2
+    size_t pstride;      // particle-source arrays: x, y, z and mass terms at
+                         // pbx + {0, 1, 2, 3} pstride
+    size_t nstride;      // node-source arrays likewise from nbx
 };
 
-// Sum over the wave's 64 lanes, complete in lane 63, by DPP moves within
-// and across rows of 16 lanes (no LDS traffic): an inclusive prefix sum in
-// each row (row_shr 1, 2, 4, 8), then row 0's and row 2's totals broadcast
-// into rows 1 and 3 (row_bcast 15) and the first half's total into rows 2
-// and 3 (row_bcast 31).  Lanes whose DPP source lies outside the row (or
-// rows outside the mask) add zero.
+constexpr int kSrcBatch = 4;
+
+// A DPP move of v (lanes whose source lies outside the masks read zero).
 template <int CTRL, int ROWS>
 __device__ __forceinline__ float dpp_f(float v)
 {
     return __builtin_bit_cast(
         float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xF, false));
-}
-
-__device__ __forceinline__ float wave_sum_lane63(float v)
-{
-    v += dpp_f<0x111, 0xF>(v); // row_shr:1
-    v += dpp_f<0x112, 0xF>(v); // row_shr:2
-    v += dpp_f<0x114, 0xF>(v); // row_shr:4
-    v += dpp_f<0x118, 0xF>(v); // row_shr:8
-    v += dpp_f<0x142, 0xA>(v); // row_bcast:15 into rows 1 and 3
-    v += dpp_f<0x143, 0xC>(v); // row_bcast:31 into rows 2 and 3
-    return v;
 }
 
 struct VjpKahan {
@@ -416,18 +518,24 @@ struct VjpLane {
     // One source for the whole wave (use false on lanes that do not
     // interact); wave-uniform control flow.
     template <bool SOFT>
-    __device__ __forceinline__ void source(bool use, const int4& sq, float eps2,
-                                           double unit_g, double unit_m, double* out_x,
-                                           double* out_y, double* out_z, double* out_m,
+    __device__ __forceinline__ void source(bool use, const int4& sq, float eps2, bool node,
                                            int index)
+    {
+        const float dx = static_cast<float>(sq.x - tq.x);
+        const float dy = static_cast<float>(sq.y - tq.y);
+        const float dz = static_cast<float>(sq.z - tq.z);
+        const float r2 = fmaf(dz, dz, fmaf(dx, dx, fmaf(dy, dy, eps2)));
+        source_at<SOFT>(use, dx, dy, dz, r2, __int_as_float(sq.w), node, index);
+    }
+
+    // The same with the separation and softened r^2 already at hand (the
+    // opening test of an internal node computed them).
+    template <bool SOFT>
+    __device__ __forceinline__ void source_at(bool use, float dx, float dy, float dz,
+                                              float r2, float gm, bool node, int index)
     {
         float gx = 0.0f, gy = 0.0f, gz = 0.0f, cm = 0.0f;
         if (use) {
-            const float dx = static_cast<float>(sq.x - tq.x);
-            const float dy = static_cast<float>(sq.y - tq.y);
-            const float dz = static_cast<float>(sq.z - tq.z);
-            const float r2 = fmaf(dz, dz, fmaf(dx, dx, fmaf(dy, dy, eps2)));
-            const float gm = __int_as_float(sq.w);
             if (!SOFT && r2 == 0.0f) {
                 if (gm != 0.0f)
                     code = 6;
@@ -447,18 +555,64 @@ struct VjpLane {
                 pz -= gz;
             }
         }
-        if (__ballot(gx != 0.0f || gy != 0.0f || gz != 0.0f || cm != 0.0f) != 0ull) {
-            gx = wave_sum_lane63(gx);
-            gy = wave_sum_lane63(gy);
-            gz = wave_sum_lane63(gz);
-            cm = wave_sum_lane63(cm);
-            if ((threadIdx.x & 63) == 63) {
-                atomicAdd(out_x + index, static_cast<double>(gx) * unit_g);
-                atomicAdd(out_y + index, static_cast<double>(gy) * unit_g);
-                atomicAdd(out_z + index, static_cast<double>(gz) * unit_g);
-                atomicAdd(out_m + index, static_cast<double>(cm) * unit_m);
-            }
+        if (__ballot(use) != 0ull)
+            park(gx, gy, gz, cm, node, index);
+    }
+
+    // ---- the wave's batch of source terms (wave-uniform calls)
+    float4* slots = nullptr; // kSrcBatch x 64 lanes
+    int2* meta = nullptr;    // per parked visit: source index, node flag
+    int parked = 0;
+    double* pbase = nullptr;
+    double* nbase = nullptr;
+    size_t pstride = 0, nstride = 0;
+    double unit_g = 0.0, unit_m = 0.0;
+
+    __device__ __forceinline__ void park(float gx, float gy, float gz, float cm, bool node,
+                                         int index)
+    {
+        const int l = threadIdx.x & 63;
+        slots[parked * 64 + l] = make_float4(gx, gy, gz, cm);
+        if (l == 0)
+            meta[parked] = make_int2(index, node ? 1 : 0);
+        if (++parked == kSrcBatch)
+            flush_sources();
+    }
+
+    // Lane l sums component (l >> 2) & 3 of parked visit l >> 4 over the
+    // 16 lanes of quarter l & 3, a quad sum adds the quarters, and the
+    // first lane of each quad adds the total to its source's accumulator.
+    __device__ __forceinline__ void flush_sources()
+    {
+        if (parked == 0)
+            return;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int l = threadIdx.x & 63;
+        const int v = l >> 4;
+        const int c = (l >> 2) & 3;
+        const int q = l & 3;
+        float sum = 0.0f;
+        if (v < parked) {
+            const float* f = reinterpret_cast<const float*>(slots + v * 64 + q * 16) + c;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                sum += f[4 * j];
         }
+        sum += dpp_f<0xB1, 0xF>(sum); // quad_perm [1, 0, 3, 2]
+        sum += dpp_f<0x4E, 0xF>(sum); // quad_perm [2, 3, 0, 1]
+        if (q == 0 && v < parked) {
+            const int2 md = meta[v];
+            double* out = md.y ? nbase + static_cast<size_t>(c) * nstride
+                               : pbase + static_cast<size_t>(c) * pstride;
+            atomicAdd(out + md.x, static_cast<double>(sum) * (c == 3 ? unit_m : unit_g));
+        }
+        // the slots are read before the next visits overwrite them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        parked = 0;
     }
 };
 
@@ -524,12 +678,22 @@ __global__ __launch_bounds__(kWalkThreads) void vjp_mixed_kernel(MixedVjp m, OpA
     L.bz = static_cast<float>(ldexp(bz, -e));
     const double ug = ldexp(m.gunit, e);
     const double um = ldexp(m.munit, e);
+    __shared__ float4 s_slots[kWalkThreads / 64][kSrcBatch * 64];
+    __shared__ int2 s_meta[kWalkThreads / 64][kSrcBatch];
+    L.slots = s_slots[threadIdx.x >> 6];
+    L.meta = s_meta[threadIdx.x >> 6];
+    L.pbase = a.pbx;
+    L.nbase = a.nbx;
+    L.pstride = m.pstride;
+    L.nstride = m.nstride;
+    L.unit_g = ug;
+    L.unit_m = um;
     int next = valid ? begin : 0x7FFFFFFF;
     int i = begin;
     unsigned cnt = 0;
     while (i < end) {
-        const int4 nd = m.wrec[2 * static_cast<size_t>(i)];
-        int4 lk = m.wrec[2 * static_cast<size_t>(i) + 1];
+        int4 nd, lk;
+        load_record32(m.wrec, i, nd, lk);
         if (m.split && (lk.z & kMixedBoundary) != 0) {
             // a marked subtree start of a split layout: its own link words
             const int subtrees = m.split[0];
@@ -548,8 +712,7 @@ __global__ __launch_bounds__(kWalkThreads) void vjp_mixed_kernel(MixedVjp m, OpA
             // always taken: a one-particle leaf (count 1, its particle) or
             // a zero-mass node (count 0, nothing)
             if ((lk.z >> 8) == 1)
-                L.source<SOFT>(active && lk.w != L.t, nd, m.eps2, ug, um, a.pbx, a.pby,
-                               a.pbz, a.pm, lk.w);
+                L.source<SOFT>(active && lk.w != L.t, nd, m.eps2, false, lk.w);
             if (active)
                 next = skip;
         } else if (kind == KIND_INTERNAL) {
@@ -569,13 +732,12 @@ __global__ __launch_bounds__(kWalkThreads) void vjp_mixed_kernel(MixedVjp m, OpA
                 open = true;
                 next = i + 1;
             }
-            L.source<SOFT>(accept, nd, m.eps2, ug, um, a.nbx, a.nby, a.nbz, a.nm, i);
+            L.source_at<SOFT>(accept, dx, dy, dz, r2, __int_as_float(nd.w), true, i);
         } else if (kind == KIND_LEAF) {
             const int count = lk.z >> 8;
             const int first = lk.w;
             for (int q = first; q < first + count; ++q)
-                L.source<SOFT>(active && q != L.t, m.rec[q], m.eps2, ug, um, a.pbx, a.pby,
-                               a.pbz, a.pm, q);
+                L.source<SOFT>(active && q != L.t, m.rec[q], m.eps2, false, q);
             if (active)
                 next = skip;
         } else if (kind == KIND_SKIP) {
@@ -595,6 +757,7 @@ __global__ __launch_bounds__(kWalkThreads) void vjp_mixed_kernel(MixedVjp m, OpA
         }
         i = (__ballot(open) != 0ull) ? i + 1 : skip;
     }
+    L.flush_sources();
     if (L.code != 0) {
         raise_error(err, L.code);
         return;
@@ -686,6 +849,10 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
     a.nbz = nbar + 2 * W;
     a.pm = pm;
     a.nm = nm;
+    // the batched source sums address component c at base + c * stride
+    if (a.pbz != a.pbx + 2 * N || a.pm != a.pbx + 3 * N || a.nbz != a.nbx + 2 * W ||
+        a.nm != a.nbx + 3 * W)
+        fail(NBH_LOGIC_ERROR, "force VJP source arrays are not laid out at fixed strides");
     launch_op<OP_VJP>(s, w64, wcenter, d_wcount, rec, idx, n, eps2, G, contains, a,
                       err);
     unsigned blocks = blocks_for(M, 256);
@@ -745,8 +912,12 @@ void launch_force_vjp_mixed(hipStream_t s, const nbh_octree_view& o, int max_lev
     if (split)
         launch_walk_split(s, wrec, d_wcount, split);
     const int units = split ? kWalkSplitMax : 1;
+    // the batched source sums address component c at base + c * stride
+    if (a.pby != a.pbx + N || a.pbz != a.pbx + 2 * N || a.pm != a.pbx + 3 * N ||
+        a.nby != a.nbx + W || a.nbz != a.nbx + 2 * W || a.nm != a.nbx + 3 * W)
+        fail(NBH_LOGIC_ERROR, "force VJP source arrays are not laid out at fixed strides");
     const MixedVjp mv{wrec, wcenter, d_wcount, split, rec32, idx, order, n, units, eps2,
-                      gunit, munit};
+                      gunit, munit, N, W};
     const unsigned blocks = blocks_for((N + 63) / 64 * units, 4);
     if (contains) {
         if (soft)
