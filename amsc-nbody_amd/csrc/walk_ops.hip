@@ -260,6 +260,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
     int next = valid ? 0 : 0x7FFFFFFF;
     int i = 0;
     while (i < wcount) {
+        i = __builtin_amdgcn_readfirstlane(i); // wave-uniform
         const Walk64Rec r = load_record64(w64, i);
         const int skip = r.skip;
         const int kind = r.kind;
@@ -632,12 +633,12 @@ __global__ __launch_bounds__(kWalkThreads) void vjp_mixed_kernel(MixedVjp m, OpA
     if (packet * 64 >= n)
         return;
     int begin = 0;
-    int end = *m.d_wcount;
+    int end = __builtin_amdgcn_readfirstlane(*m.d_wcount);
     if (m.split) {
-        if (sub >= m.split[0])
+        if (sub >= __builtin_amdgcn_readfirstlane(m.split[0]))
             return;
-        begin = m.split[1 + 4 * sub];
-        end = m.split[2 + 4 * sub];
+        begin = __builtin_amdgcn_readfirstlane(m.split[1 + 4 * sub]);
+        end = __builtin_amdgcn_readfirstlane(m.split[2 + 4 * sub]);
     }
     const int k = packet * 64 + lane;
     bool valid = k < n;
@@ -692,15 +693,17 @@ __global__ __launch_bounds__(kWalkThreads) void vjp_mixed_kernel(MixedVjp m, OpA
     int i = begin;
     unsigned cnt = 0;
     while (i < end) {
+        i = __builtin_amdgcn_readfirstlane(i); // wave-uniform
         int4 nd, lk;
         load_record32(m.wrec, i, nd, lk);
         if (m.split && (lk.z & kMixedBoundary) != 0) {
             // a marked subtree start of a split layout: its own link words
-            const int subtrees = m.split[0];
+            // (read as wave-uniform values, like the record)
+            const int subtrees = __builtin_amdgcn_readfirstlane(m.split[0]);
             for (int q = 0; q < subtrees; ++q)
-                if (m.split[1 + 4 * q] == i) {
-                    lk.x = m.split[3 + 4 * q];
-                    lk.z = m.split[4 + 4 * q];
+                if (__builtin_amdgcn_readfirstlane(m.split[1 + 4 * q]) == i) {
+                    lk.x = __builtin_amdgcn_readfirstlane(m.split[3 + 4 * q]);
+                    lk.z = __builtin_amdgcn_readfirstlane(m.split[4 + 4 * q]);
                 }
         }
         const int skip = lk.y;
