@@ -330,14 +330,18 @@ void launch_walk_split(hipStream_t s, int4* wpairs, const int* d_wcount, int* sp
 // order: sorted target indices in the packets' order (null: Morton order).
 // wpairs: the {node, link} record pairs; a split walk marks its subtrees'
 // first records in place (see walk_split_kernel).
+// perm, pcost (packets entries each, whole walks only): a whole walk hands
+// out its packets in the order built from pcost (packet_schedule_kernel)
+// and, unless it counts interactions, records their new visit counts in
+// pcost.  Zero costs give the curve order.
 void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx,
                        const uint32_t* order, int n,
                        float eps2, double out_scale, bool contains, bool soft,
                        double* ax, double* ay, double* az, int* queue,
-                       int* split, double* part, int* err,
-                       unsigned long long* counters);
+                       int* split, double* part, uint32_t* perm, uint32_t* pcost,
+                       int* err, unsigned long long* counters);
 void launch_direct(hipStream_t s, const double* m, const double* x,
                    const double* y, const double* z, size_t n,
                    const uint32_t* targets, size_t n_targets, double eps2,

@@ -94,6 +94,107 @@ __global__ void queue_clear_kernel(int* __restrict__ queue)
         queue[threadIdx.x * kQueueStride] = 0;
 }
 
+// The order in which the queue hands out the packets of a whole-walk
+// traversal, from each packet's visit count in the previous traversal of
+// the same particles (packet k then held nearly the same targets along the
+// curve).  Within each of the eight queue ranges, the packets that cost
+// more than the range's median come first, in curve order; the others
+// follow in decreasing cost.  The last packets a range hands out are then
+// its cheapest, so the slots run out of work close together instead of
+// waiting on packets taken just before the queue emptied.  Costs of zero
+// (no previous traversal) leave the curve order.  perm[q] is the packet
+// handed out at queue position q; packets within one cost bin of the cheap
+// part may come in any order.  One block of kScheduleThreads per range.
+constexpr int kScheduleThreads = 1024;
+constexpr int kScheduleBins = 256;
+
+__global__ __launch_bounds__(kScheduleThreads) void packet_schedule_kernel(
+    const uint32_t* __restrict__ cost, int packets, uint32_t* __restrict__ perm)
+{
+    __shared__ unsigned s_bin[kScheduleBins]; // histogram, then cursors
+    __shared__ unsigned s_wave[kScheduleThreads / 64];
+    __shared__ unsigned s_cut[2];             // median bin, expensive count
+    const int per = (packets + 7) / 8;
+    const int lo = static_cast<int>(blockIdx.x) * per;
+    const int hi = min(lo + per, packets);
+    if (lo >= hi)
+        return;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    unsigned mx = 0;
+    for (int k = lo + tid; k < hi; k += kScheduleThreads)
+        mx = max(mx, cost[k]);
+    for (int o = 32; o > 0; o >>= 1)
+        mx = max(mx, static_cast<unsigned>(__shfl_xor(static_cast<int>(mx), o)));
+    if (lane == 0)
+        s_wave[wave] = mx;
+    for (int b = tid; b < kScheduleBins; b += kScheduleThreads)
+        s_bin[b] = 0;
+    __syncthreads();
+    mx = 0;
+    for (int w = 0; w < kScheduleThreads / 64; ++w)
+        mx = max(mx, s_wave[w]);
+    if (mx == 0) {
+        for (int k = lo + tid; k < hi; k += kScheduleThreads)
+            perm[k] = static_cast<uint32_t>(k);
+        return;
+    }
+    auto bin_of = [mx](unsigned c) {
+        return static_cast<int>((static_cast<unsigned long long>(c) * kScheduleBins) /
+                                (static_cast<unsigned long long>(mx) + 1));
+    };
+    for (int k = lo + tid; k < hi; k += kScheduleThreads)
+        atomicAdd(&s_bin[bin_of(cost[k])], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        // the median bin: the cheap part is bins 0..cut, laid out after the
+        // expensive part from bin cut down to bin 0
+        const unsigned half = static_cast<unsigned>(hi - lo + 1) / 2;
+        unsigned below = 0;
+        int cut = 0;
+        while (cut < kScheduleBins - 1 && below + s_bin[cut] < half)
+            below += s_bin[cut++];
+        below += s_bin[cut];
+        const unsigned expensive = static_cast<unsigned>(hi - lo) - below;
+        unsigned at = expensive;
+        for (int b = cut; b >= 0; --b) {
+            const unsigned c = s_bin[b];
+            s_bin[b] = at;
+            at += c;
+        }
+        s_cut[0] = static_cast<unsigned>(cut);
+        s_cut[1] = expensive;
+    }
+    __syncthreads();
+    const int cut = static_cast<int>(s_cut[0]);
+    unsigned base = 0; // expensive packets placed by earlier chunks
+    for (int c0 = lo; c0 < hi; c0 += kScheduleThreads) {
+        const int k = c0 + tid;
+        const int b = k < hi ? bin_of(cost[k]) : -1;
+        const bool dear = b > cut;
+        const unsigned long long m = __ballot(dear);
+        const unsigned before = static_cast<unsigned>(
+            __popcll(m & ((1ull << lane) - 1ull)));
+        if (lane == 0)
+            s_wave[wave] = static_cast<unsigned>(__popcll(m));
+        __syncthreads();
+        unsigned off = base;
+        unsigned total = 0;
+        for (int w = 0; w < kScheduleThreads / 64; ++w) {
+            const unsigned c = s_wave[w];
+            off += w < wave ? c : 0u;
+            total += c;
+        }
+        if (dear)
+            perm[lo + off + before] = static_cast<uint32_t>(k);
+        else if (b >= 0)
+            perm[lo + atomicAdd(&s_bin[b], 1u)] = static_cast<uint32_t>(k);
+        base += total;
+        __syncthreads();
+    }
+}
+
 } // namespace
 
 // Blocks of 4 persistent wavefronts: one wavefront per unit_per_wave units

@@ -137,7 +137,10 @@ struct Walker {
     int i;         // current position (wave-uniform)
     int end;       // end of the unit's positions (the walk length if whole)
     int unit;      // the unit walked (split walks: its partial-sum slot)
-    unsigned cnt;  // visits since the packet's last fold (wave-uniform)
+    int pk;        // the packet walked (whole walks: its visit count is kept)
+    // Visits of the packet's walk (wave-uniform): those since the last fold
+    // in the bits below kFold, the folds from bit 2 * kFold up.
+    unsigned cnt;
     Acc a;
     unsigned long long npp, npn, visits;
 
@@ -234,13 +237,20 @@ __device__ __forceinline__ bool lane_bit(unsigned long long m)
 }
 
 // Ends a visit: folds the partial sums after the packet's kFold-th visit
-// since the last fold (`cnt` was incremented for this visit).
+// since the last fold (`cnt` was incremented for this visit); adding kFold
+// then carries the kFold bit into the fold count.
 __device__ __forceinline__ void end_visit(Walker& w)
 {
-    if (w.cnt == kFold) {
+    if ((w.cnt & kFold) != 0u) {
         w.a.fold();
-        w.cnt = 0;
+        w.cnt += kFold;
     }
+}
+
+// The visits of a packet's walk from its visit counter.
+__device__ __forceinline__ unsigned packet_visits(unsigned cnt)
+{
+    return (cnt / (2 * kFold)) * kFold + (cnt & (kFold - 1));
 }
 
 // The straight-line part of a visit, for records of every kind: the MAC
@@ -484,14 +494,19 @@ __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__
 
 // A whole walk writes the target's acceleration; a split unit (part given)
 // writes its FP64 partial sums at part[(unit * 64 + lane) * 3].
+// A whole walk records its packet's visit count in pcost (a split walk has
+// no schedule; a counting walk leaves the costs).
 template <bool COUNT, bool SPLIT>
 __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ idx,
                                        int n, double out_scale,
                                        double* __restrict__ ax_out,
                                        double* __restrict__ ay_out,
                                        double* __restrict__ az_out,
-                                       double* __restrict__ part, int* err)
+                                       double* __restrict__ part,
+                                       uint32_t* __restrict__ pcost, int* err)
 {
+    if (!SPLIT && !COUNT && (threadIdx.x & 63) == 0)
+        pcost[w.pk] = packet_visits(w.cnt);
     w.a.fold();
     if (w.code != 0) {
         raise_error(err, w.code);
@@ -526,7 +541,9 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
 // leaves once the queue is empty and its slots are done.
 // SPLIT (split and part given): unit u walks subtree u % s of packet u / s,
 // with split = {s, then per subtree {first, end, own crit^2, own kind
-// word}}; only these instantiations handle boundary records.
+// word}}; only these instantiations handle boundary records.  Whole walks
+// take packet perm[u] at queue position u and record each packet's visit
+// count in pcost for the next schedule (counting walks leave pcost).
 template <bool CONTAINS, bool SOFT, bool COUNT, bool WIDE, bool SPLIT>
 __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const WalkRec* __restrict__ wrec, unsigned wbytes,
@@ -536,7 +553,8 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     int* __restrict__ queue, float eps2, double out_scale,
     double* __restrict__ ax_out,
     double* __restrict__ ay_out, double* __restrict__ az_out,
-    const int* __restrict__ split, double* __restrict__ part, int* err,
+    const int* __restrict__ split, double* __restrict__ part,
+    const uint32_t* __restrict__ perm, uint32_t* __restrict__ pcost, int* err,
     unsigned long long* __restrict__ counters)
 {
     const int lane = threadIdx.x & 63;
@@ -549,6 +567,12 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
         return;
     const int wcount = *d_wcount;
     const __amdgpu_buffer_rsrc_t rsrc = records_rsrc(wrec, WIDE ? 0u : wbytes);
+    // The schedule's pointers and packet numbers live in vector registers:
+    // they are used once per packet, and the scalar registers of the walk
+    // are spent.
+    const uint32_t* permv = perm;
+    uint32_t* pcostv = pcost;
+    asm volatile("" : "+v"(permv), "+v"(pcostv));
     auto begin = [&](Walker& w, int u) {
         if (SPLIT) {
             const int p = u / subtrees;
@@ -560,7 +584,12 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             w.end = split[2 + 4 * k];
             w.unit = u;
         } else {
-            w.start(rec, order, u, lane, n);
+            // perm is a permutation by construction; the bound keeps
+            // every access inside the targets whatever it holds
+            int p = static_cast<int>(min(permv[u], static_cast<uint32_t>(packets)));
+            asm volatile("" : "+v"(p));
+            w.start(rec, order, p, lane, n);
+            w.pk = p;
         }
     };
     Walker A, B;
@@ -624,7 +653,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             visit_rare<SOFT, COUNT>(A, ra, rec, eps2);
             end_visit(A);
         } else {
-            finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+            finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, pcostv, err);
             const int u = q.take();
             if (u >= 0)
                 begin(A, u);
@@ -647,7 +676,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
             visit_rare<SOFT, COUNT>(B, rb, rec, eps2);
             end_visit(B);
         } else {
-            finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+            finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, pcostv, err);
             const int u = q.take();
             if (u >= 0)
                 begin(B, u);
@@ -657,11 +686,11 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     }
     if (run_a) {
         walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(A, wrec, wcenter, rec, split, wcount, eps2);
-        finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+        finish<COUNT, SPLIT>(A, idx, n, out_scale, ax_out, ay_out, az_out, part, pcostv, err);
     }
     if (run_b) {
         walk_rest<CONTAINS, SOFT, COUNT, SPLIT>(B, wrec, wcenter, rec, split, wcount, eps2);
-        finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, err);
+        finish<COUNT, SPLIT>(B, idx, n, out_scale, ax_out, ay_out, az_out, part, pcostv, err);
     }
     if (COUNT) {
         unsigned long long npp = A.npp + B.npp, npn = A.npn + B.npn;
@@ -787,8 +816,8 @@ void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
                        const uint32_t* order, int n,
                        float eps2, double out_scale, bool contains, bool soft,
                        double* ax, double* ay, double* az, int* queue,
-                       int* split, double* part, int* err,
-                       unsigned long long* counters)
+                       int* split, double* part, uint32_t* perm, uint32_t* pcost,
+                       int* err, unsigned long long* counters)
 {
     // 32-bit record offsets while the record array spans less than 2 GiB
     const size_t wbytes = wcap * sizeof(WalkRec);
@@ -798,12 +827,20 @@ void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
         fail(NBH_LOGIC_ERROR, "a split traversal needs fewer packets and a partial-sum buffer");
     queue_clear_kernel<<<1, 64, 0, s>>>(queue);
     NBH_HIP(hipGetLastError());
+    const long long packets = (static_cast<long long>(n) + 63) / 64;
+    // whole walks hand out their packets in the order built from the costs
+    if (!splits && (!perm || !pcost))
+        fail(NBH_LOGIC_ERROR, "a whole traversal needs its packet schedule buffers");
+    if (!splits && packets > 0) {
+        packet_schedule_kernel<<<8, kScheduleThreads, 0, s>>>(pcost, static_cast<int>(packets),
+                                                             perm);
+        NBH_HIP(hipGetLastError());
+    }
     auto* wrec = reinterpret_cast<WalkRec*>(wpairs);
     if (splits) {
         walk_split_kernel<<<1, 64, 0, s>>>(wrec, d_wcount, split);
         NBH_HIP(hipGetLastError());
     }
-    const long long packets = (static_cast<long long>(n) + 63) / 64;
     // Whole walks: one pair of packets per wavefront, 8 resident per SIMD.
     // Split walks: about eight subtree units per wavefront; their short
     // units run faster with fewer wavefronts (100K particles: 1.25 ms per
@@ -816,7 +853,8 @@ void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
 #define NBH_ARGS                                                                 \
     wrec, static_cast<unsigned>(wide ? 0 : wbytes), wcenter, d_wcount, rec, idx, \
         order, n, queue, eps2, out_scale, ax, ay, az, splits ? split : nullptr, \
-        splits ? part : nullptr, err, counters
+        splits ? part : nullptr, splits ? nullptr : perm, splits ? nullptr : pcost, err, \
+        counters
 #define NBH_LAUNCH(C, S, K)                                                      \
     do {                                                                         \
         if (splits)                                                              \

@@ -165,6 +165,31 @@ def test_mixed_walk_independent_of_schedule(nb, n):
         s.close()
 
 
+@pytest.mark.parametrize("kind,n", [("clumps", 300_000), ("plummer", 1_000_000)])
+def test_mixed_walk_cost_ordered_queue_covers_every_packet(nb, kind, n):
+    """Whole walks hand out their packets in an order built from each
+    packet's visits in the previous traversal.  After a few steps the last
+    traversal ran in such an order; a new simulation of the same state runs
+    its first traversal in curve order.  Both walk the same tree, so every
+    target's acceleration agrees bit for bit: a packet left out or walked
+    twice by the ordered queue would leave a stale or different value."""
+    p = make(nb, kind, n, seed=5)
+    sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
+                        precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    sim.initialize()
+    sim.run(3)
+    a = sim.accelerations()
+    st = sim.state()
+    sim.close()
+    fresh = nb.Simulation(st, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
+                          precision=nb.PRECISION_MIXED, bits_per_axis=21)
+    fresh.initialize()
+    b = fresh.accelerations()
+    fresh.close()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("kind,pos_scale,mass,G,eps", [
     ("blob", 1.0, 1e30, 6.6743e-11, 1e-6),    # SI units: G m s^2 beyond FP32
     ("plummer", 1e15, 1.0, 1.0, 1e-6),        # eps^2 in grid units ~1e-24
