@@ -97,23 +97,24 @@ __global__ void queue_clear_kernel(int* __restrict__ queue)
 // The order in which the queue hands out the packets of a whole-walk
 // traversal, from each packet's visit count in the previous traversal of
 // the same particles (packet k then held nearly the same targets along the
-// curve).  Within each of the eight queue ranges, the packets that cost
-// more than the range's median come first, in curve order; the others
-// follow in decreasing cost.  The last packets a range hands out are then
-// its cheapest, so the slots run out of work close together instead of
-// waiting on packets taken just before the queue emptied.  Costs of zero
-// (no previous traversal) leave the curve order.  perm[q] is the packet
-// handed out at queue position q; packets within one cost bin of the cheap
-// part may come in any order.  One block of kScheduleThreads per range.
+// curve).  Each of the eight queue ranges ends with its `tail` cheapest
+// packets (to the cost bin) in decreasing cost; the others come first, in
+// curve order.  The last packets a range hands out are then its cheapest,
+// so the slots run out of work close together instead of waiting on
+// packets taken just before the queue emptied, while most packets keep the
+// curve order that the slots of an XCD share records by.  Costs of zero (no
+// previous traversal) or tail = 0 leave the curve order.  perm[q] is the
+// packet handed out at queue position q; packets within one cost bin of the
+// tail may come in any order.  One block of kScheduleThreads per range.
 constexpr int kScheduleThreads = 1024;
 constexpr int kScheduleBins = 256;
 
 __global__ __launch_bounds__(kScheduleThreads) void packet_schedule_kernel(
-    const uint32_t* __restrict__ cost, int packets, uint32_t* __restrict__ perm)
+    const uint32_t* __restrict__ cost, int packets, int tail, uint32_t* __restrict__ perm)
 {
     __shared__ unsigned s_bin[kScheduleBins]; // histogram, then cursors
     __shared__ unsigned s_wave[kScheduleThreads / 64];
-    __shared__ unsigned s_cut[2];             // median bin, expensive count
+    __shared__ unsigned s_cut[2];             // last tail bin, packets before the tail
     const int per = (packets + 7) / 8;
     const int lo = static_cast<int>(blockIdx.x) * per;
     const int hi = min(lo + per, packets);
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(kScheduleThreads) void packet_schedule_kernel(
     mx = 0;
     for (int w = 0; w < kScheduleThreads / 64; ++w)
         mx = max(mx, s_wave[w]);
-    if (mx == 0) {
+    if (mx == 0 || tail <= 0) {
         for (int k = lo + tid; k < hi; k += kScheduleThreads)
             perm[k] = static_cast<uint32_t>(k);
         return;
@@ -148,32 +149,32 @@ __global__ __launch_bounds__(kScheduleThreads) void packet_schedule_kernel(
         atomicAdd(&s_bin[bin_of(cost[k])], 1u);
     __syncthreads();
     if (tid == 0) {
-        // the median bin: the cheap part is bins 0..cut, laid out after the
-        // expensive part from bin cut down to bin 0
-        const unsigned half = static_cast<unsigned>(hi - lo + 1) / 2;
+        // the tail is bins 0..cut, the lowest that hold `tail` packets,
+        // laid out after the other packets from bin cut down to bin 0
+        const unsigned want = static_cast<unsigned>(min(tail, hi - lo));
         unsigned below = 0;
         int cut = 0;
-        while (cut < kScheduleBins - 1 && below + s_bin[cut] < half)
+        while (cut < kScheduleBins - 1 && below + s_bin[cut] < want)
             below += s_bin[cut++];
         below += s_bin[cut];
-        const unsigned expensive = static_cast<unsigned>(hi - lo) - below;
-        unsigned at = expensive;
+        const unsigned head = static_cast<unsigned>(hi - lo) - below;
+        unsigned at = head;
         for (int b = cut; b >= 0; --b) {
             const unsigned c = s_bin[b];
             s_bin[b] = at;
             at += c;
         }
         s_cut[0] = static_cast<unsigned>(cut);
-        s_cut[1] = expensive;
+        s_cut[1] = head;
     }
     __syncthreads();
     const int cut = static_cast<int>(s_cut[0]);
-    unsigned base = 0; // expensive packets placed by earlier chunks
+    unsigned base = 0; // head packets placed by earlier chunks
     for (int c0 = lo; c0 < hi; c0 += kScheduleThreads) {
         const int k = c0 + tid;
         const int b = k < hi ? bin_of(cost[k]) : -1;
-        const bool dear = b > cut;
-        const unsigned long long m = __ballot(dear);
+        const bool first = b > cut; // before the tail, in curve order
+        const unsigned long long m = __ballot(first);
         const unsigned before = static_cast<unsigned>(
             __popcll(m & ((1ull << lane) - 1ull)));
         if (lane == 0)
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(kScheduleThreads) void packet_schedule_kernel(
             off += w < wave ? c : 0u;
             total += c;
         }
-        if (dear)
+        if (first)
             perm[lo + off + before] = static_cast<uint32_t>(k);
         else if (b >= 0)
             perm[lo + atomicAdd(&s_bin[b], 1u)] = static_cast<uint32_t>(k);

@@ -828,14 +828,6 @@ void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
     queue_clear_kernel<<<1, 64, 0, s>>>(queue);
     NBH_HIP(hipGetLastError());
     const long long packets = (static_cast<long long>(n) + 63) / 64;
-    // whole walks hand out their packets in the order built from the costs
-    if (!splits && (!perm || !pcost))
-        fail(NBH_LOGIC_ERROR, "a whole traversal needs its packet schedule buffers");
-    if (!splits && packets > 0) {
-        packet_schedule_kernel<<<8, kScheduleThreads, 0, s>>>(pcost, static_cast<int>(packets),
-                                                             perm);
-        NBH_HIP(hipGetLastError());
-    }
     auto* wrec = reinterpret_cast<WalkRec*>(wpairs);
     if (splits) {
         walk_split_kernel<<<1, 64, 0, s>>>(wrec, d_wcount, split);
@@ -850,6 +842,22 @@ void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
                           : walk_persistent_blocks(packets, 2, kWalkMixedWavesPerSimd);
     if (nb == 0)
         return;
+    // Whole walks hand out their packets in the order built from the costs:
+    // each range ends with as many of its cheapest packets as the range has
+    // slots (two per wavefront), the last ones started.  With no more
+    // packets than slots every packet starts at once and the curve order
+    // stays (reordering then only scatters the packets that the slots of a
+    // wavefront and of an XCD walk together: 1M uniform particles, 16,384
+    // packets, took 3.5 % longer).
+    if (!splits && (!perm || !pcost))
+        fail(NBH_LOGIC_ERROR, "a whole traversal needs its packet schedule buffers");
+    if (!splits) {
+        const long long slots = static_cast<long long>(nb) * 4 * 2;
+        const int tail = packets > slots ? static_cast<int>((slots + 7) / 8) : 0;
+        packet_schedule_kernel<<<8, kScheduleThreads, 0, s>>>(pcost, static_cast<int>(packets),
+                                                             tail, perm);
+        NBH_HIP(hipGetLastError());
+    }
 #define NBH_ARGS                                                                 \
     wrec, static_cast<unsigned>(wide ? 0 : wbytes), wcenter, d_wcount, rec, idx, \
         order, n, queue, eps2, out_scale, ax, ay, az, splits ? split : nullptr, \

@@ -165,14 +165,16 @@ def test_mixed_walk_independent_of_schedule(nb, n):
         s.close()
 
 
-@pytest.mark.parametrize("kind,n", [("clumps", 300_000), ("plummer", 1_000_000)])
+@pytest.mark.parametrize("kind,n", [("clumps", 1_500_000), ("plummer", 2_097_152)])
 def test_mixed_walk_cost_ordered_queue_covers_every_packet(nb, kind, n):
-    """Whole walks hand out their packets in an order built from each
-    packet's visits in the previous traversal.  After a few steps the last
-    traversal ran in such an order; a new simulation of the same state runs
-    its first traversal in curve order.  Both walk the same tree, so every
-    target's acceleration agrees bit for bit: a packet left out or walked
-    twice by the ordered queue would leave a stale or different value."""
+    """Whole walks with more packets than the device has slots (two per
+    resident wavefront: 16,384 on 256 CUs) hand out their packets in an
+    order built from each packet's visits in the previous traversal.  After
+    a few steps the last traversal ran in such an order; a new simulation of
+    the same state runs its first traversal in curve order.  Both walk the
+    same tree, so every target's acceleration agrees bit for bit: a packet
+    left out or walked twice by the ordered queue would leave a stale or
+    different value."""
     p = make(nb, kind, n, seed=5)
     sim = nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
                         precision=nb.PRECISION_MIXED, bits_per_axis=21)
