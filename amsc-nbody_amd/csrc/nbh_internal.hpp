@@ -330,10 +330,11 @@ void launch_walk_split(hipStream_t s, int4* wpairs, const int* d_wcount, int* sp
 // order: sorted target indices in the packets' order (null: Morton order).
 // wpairs: the {node, link} record pairs; a split walk marks its subtrees'
 // first records in place (see walk_split_kernel).
-// perm, pcost (packets entries each, whole walks only): a whole walk hands
-// out its packets in the order built from pcost (packet_schedule_kernel)
-// and, unless it counts interactions, records their new visit counts in
-// pcost.  Zero costs give the curve order.
+// perm, pcost (one entry per unit each: packets, or packets times
+// kWalkSplitMax for a split walk): the walk hands out its units in the
+// order built from pcost (packet_schedule_kernel) and, unless it counts
+// interactions, records their new visit counts in pcost.  Zero costs give
+// the curve order.
 void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
                        const int4* wcenter, const int* d_wcount,
                        const int4* rec, const uint32_t* idx,

@@ -160,8 +160,8 @@ struct nbh_sim {
     nbh::DevBuf<int> scalars;     // [3] wcount ([0..2] unused)
     nbh::DevBuf<unsigned long long> counters;
     nbh::DevBuf<int> walkq;       // mixed traversal work-queue counters
-    // whole-walk packet schedule: the queue order, then each packet's cost
-    // in the last traversal (zeroed at creation)
+    // mixed traversal schedule: the queue order of its units, then each
+    // unit's cost in the last traversal (zeroed at creation)
     nbh::DevBuf<uint32_t> psched;
     // mixed traversal split into subtree units (few packets only): the
     // units' position ranges and their partial sums

@@ -60,11 +60,12 @@ void alloc_state(nbh_sim* s)
             s->wpart.alloc((n + 63) / 64 * kWalkSplitMax * 64 * 3);
         }
     }
-    if (mixed(s) && !walk_mixed_splits(static_cast<int>(n))) {
-        const size_t packets = (n + 63) / 64;
-        s->psched.alloc(2 * packets);
-        NBH_HIP(hipMemsetAsync(s->psched.p + packets, 0, packets * sizeof(uint32_t),
-                               s->stream));
+    if (mixed(s)) {
+        // the traversal's units: packets, or packets times subtrees if split
+        const size_t units = (n + 63) / 64 * (walk_mixed_splits(static_cast<int>(n))
+                                                  ? kWalkSplitMax : 1);
+        s->psched.alloc(2 * units);
+        NBH_HIP(hipMemsetAsync(s->psched.p + units, 0, units * sizeof(uint32_t), s->stream));
     }
     s->gfirst.alloc(n);
     s->gcount.alloc(n);
@@ -255,7 +256,7 @@ void forces(nbh_sim* s, int buffer, unsigned long long* counters)
                           static_cast<float>(mixed_eps2(eps, scale)), out_scale, contains,
                           eps > 0.0,
                           a[0].p, a[1].p, a[2].p, s->walkq.p, s->wsplit.p, s->wpart.p,
-                          s->psched.p, s->psched.p + (s->n + 63) / 64,
+                          s->psched.p, s->psched.p + s->psched.n / 2,
                           s->errors + ERR_TRAVERSAL, counters);
     } else {
         launch_walk_fp64(st, s->w64.p, s->mcap, s->w64center.p, s->scalars.p + 3,

@@ -94,30 +94,33 @@ __global__ void queue_clear_kernel(int* __restrict__ queue)
         queue[threadIdx.x * kQueueStride] = 0;
 }
 
-// The order in which the queue hands out the packets of a whole-walk
-// traversal, from each packet's visit count in the previous traversal of
-// the same particles (packet k then held nearly the same targets along the
-// curve).  Each of the eight queue ranges ends with its `tail` cheapest
-// packets (to the cost bin) in decreasing cost; the others come first, in
-// curve order.  The last packets a range hands out are then its cheapest,
-// so the slots run out of work close together instead of waiting on
-// packets taken just before the queue emptied, while most packets keep the
-// curve order that the slots of an XCD share records by.  Costs of zero (no
-// previous traversal) or tail = 0 leave the curve order.  perm[q] is the
-// packet handed out at queue position q; packets within one cost bin of the
-// tail may come in any order.  One block of kScheduleThreads per range.
+// The order in which the queue hands out the units of a traversal (its
+// packets, or with a split table packets times subtrees), from each unit's
+// visit count in the previous traversal of the same particles (unit k then
+// held nearly the same targets along the curve).  Each of the eight queue
+// ranges ends with its `tail` cheapest units (to the cost bin) in
+// decreasing cost; the others come first, in curve order.  The last units
+// a range hands out are then its cheapest, so the slots run out of work
+// close together instead of waiting on units taken just before the queue
+// emptied, while the others keep the curve order that the slots of an XCD
+// share records by.  Costs of zero (no previous traversal) or tail = 0
+// leave the curve order.  perm[q] is the unit handed out at queue position
+// q; units within one cost bin of the tail may come in any order.  One
+// block of kScheduleThreads per range.
 constexpr int kScheduleThreads = 1024;
 constexpr int kScheduleBins = 256;
 
 __global__ __launch_bounds__(kScheduleThreads) void packet_schedule_kernel(
-    const uint32_t* __restrict__ cost, int packets, int tail, uint32_t* __restrict__ perm)
+    const uint32_t* __restrict__ cost, int packets, const int* __restrict__ split, int tail,
+    uint32_t* __restrict__ perm)
 {
     __shared__ unsigned s_bin[kScheduleBins]; // histogram, then cursors
     __shared__ unsigned s_wave[kScheduleThreads / 64];
     __shared__ unsigned s_cut[2];             // last tail bin, packets before the tail
-    const int per = (packets + 7) / 8;
+    const int units = split ? packets * split[0] : packets;
+    const int per = (units + 7) / 8;
     const int lo = static_cast<int>(blockIdx.x) * per;
-    const int hi = min(lo + per, packets);
+    const int hi = min(lo + per, units);
     if (lo >= hi)
         return;
     const int tid = static_cast<int>(threadIdx.x);

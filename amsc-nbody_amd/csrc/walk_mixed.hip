@@ -494,8 +494,8 @@ __device__ __forceinline__ void walk_rest(Walker& w, const WalkRec* __restrict__
 
 // A whole walk writes the target's acceleration; a split unit (part given)
 // writes its FP64 partial sums at part[(unit * 64 + lane) * 3].
-// A whole walk records its packet's visit count in pcost (a split walk has
-// no schedule; a counting walk leaves the costs).
+// Each unit's visit count goes to pcost for the next schedule (a counting
+// walk leaves the costs).
 template <bool COUNT, bool SPLIT>
 __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ idx,
                                        int n, double out_scale,
@@ -505,8 +505,8 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
                                        double* __restrict__ part,
                                        uint32_t* __restrict__ pcost, int* err)
 {
-    if (!SPLIT && !COUNT && (threadIdx.x & 63) == 0)
-        pcost[w.pk] = packet_visits(w.cnt);
+    if (!COUNT && (threadIdx.x & 63) == 0)
+        pcost[SPLIT ? w.unit : w.pk] = packet_visits(w.cnt);
     w.a.fold();
     if (w.code != 0) {
         raise_error(err, w.code);
@@ -541,9 +541,9 @@ __device__ __forceinline__ void finish(Walker& w, const uint32_t* __restrict__ i
 // leaves once the queue is empty and its slots are done.
 // SPLIT (split and part given): unit u walks subtree u % s of packet u / s,
 // with split = {s, then per subtree {first, end, own crit^2, own kind
-// word}}; only these instantiations handle boundary records.  Whole walks
-// take packet perm[u] at queue position u and record each packet's visit
-// count in pcost for the next schedule (counting walks leave pcost).
+// word}}; only these instantiations handle boundary records.  Queue
+// position q holds unit perm[q]; each unit's visit count goes to pcost for
+// the next schedule (counting walks leave pcost).
 template <bool CONTAINS, bool SOFT, bool COUNT, bool WIDE, bool SPLIT>
 __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const WalkRec* __restrict__ wrec, unsigned wbytes,
@@ -573,8 +573,12 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
     const uint32_t* permv = perm;
     uint32_t* pcostv = pcost;
     asm volatile("" : "+v"(permv), "+v"(pcostv));
-    auto begin = [&](Walker& w, int u) {
+    auto begin = [&](Walker& w, int q) {
         if (SPLIT) {
+            // perm is a permutation by construction; the bound keeps
+            // every access inside the units whatever it holds
+            const int u = __builtin_amdgcn_readfirstlane(
+                static_cast<int>(min(permv[q], static_cast<uint32_t>(units - 1))));
             const int p = u / subtrees;
             const int k = u - p * subtrees;
             w.start(rec, order, p, lane, n);
@@ -586,7 +590,7 @@ __global__ __launch_bounds__(kWalkThreads) void walk_mixed_kernel(
         } else {
             // perm is a permutation by construction; the bound keeps
             // every access inside the targets whatever it holds
-            int p = static_cast<int>(min(permv[u], static_cast<uint32_t>(packets)));
+            int p = static_cast<int>(min(permv[q], static_cast<uint32_t>(packets)));
             asm volatile("" : "+v"(p));
             w.start(rec, order, p, lane, n);
             w.pk = p;
@@ -834,34 +838,39 @@ void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
         NBH_HIP(hipGetLastError());
     }
     // Whole walks: one pair of packets per wavefront, 8 resident per SIMD.
-    // Split walks: about eight subtree units per wavefront; their short
-    // units run faster with fewer wavefronts (100K particles: 1.25 ms per
-    // step at 2 per SIMD against 1.40 ms at 8; 250K: 1.63 ms at 4 against
-    // 1.82 ms at 8).
-    const int nb = splits ? walk_persistent_blocks(packets * kWalkSplitMax, 8, 8)
+    // Split walks: at least four subtree units per wavefront and at most 2
+    // wavefronts per SIMD; their short units run faster with few wavefronts
+    // (SI blob, cost-ordered units, step time: 250K particles 1.42 ms at 2
+    // per SIMD against 1.50 at 3 and 1.58 at 4; 100K 1.21 ms, 30K 0.97 ms).
+    const int nb = splits ? walk_persistent_blocks(packets * kWalkSplitMax, 4, 2)
                           : walk_persistent_blocks(packets, 2, kWalkMixedWavesPerSimd);
     if (nb == 0)
         return;
-    // Whole walks hand out their packets in the order built from the costs:
-    // each range ends with as many of its cheapest packets as the range has
-    // slots (two per wavefront), the last ones started.  With no more
-    // packets than slots every packet starts at once and the curve order
-    // stays (reordering then only scatters the packets that the slots of a
-    // wavefront and of an XCD walk together: 1M uniform particles, 16,384
-    // packets, took 3.5 % longer).
-    if (!splits && (!perm || !pcost))
-        fail(NBH_LOGIC_ERROR, "a whole traversal needs its packet schedule buffers");
-    if (!splits) {
+    // The queue hands out the units in the order built from their costs.
+    // Whole walks: each range ends with as many of its cheapest packets as
+    // the range has slots (two per wavefront), the last ones started, the
+    // others in curve order.  With no more packets than slots every packet
+    // starts at once and the curve order stays (reordering then only
+    // scatters the packets that the slots of a wavefront and of an XCD walk
+    // together: 1M uniform particles, 16,384 packets, took 3.5 % longer).
+    // Split walks, whose units differ most in cost (the subtree holding the
+    // packet's own targets carries most of its near field) and whose records
+    // fit the caches: every unit in decreasing cost (SI blob, 100K: 1.30 ->
+    // 1.21 ms per step; 250K: 1.68 -> 1.58 ms).
+    if (!perm || !pcost)
+        fail(NBH_LOGIC_ERROR, "a traversal needs its unit schedule buffers");
+    {
         const long long slots = static_cast<long long>(nb) * 4 * 2;
-        const int tail = packets > slots ? static_cast<int>((slots + 7) / 8) : 0;
-        packet_schedule_kernel<<<8, kScheduleThreads, 0, s>>>(pcost, static_cast<int>(packets),
-                                                             tail, perm);
+        const int tail = splits ? static_cast<int>(packets * kWalkSplitMax)
+                                : (packets > slots ? static_cast<int>((slots + 7) / 8) : 0);
+        packet_schedule_kernel<<<8, kScheduleThreads, 0, s>>>(
+            pcost, static_cast<int>(packets), splits ? split : nullptr, tail, perm);
         NBH_HIP(hipGetLastError());
     }
 #define NBH_ARGS                                                                 \
     wrec, static_cast<unsigned>(wide ? 0 : wbytes), wcenter, d_wcount, rec, idx, \
         order, n, queue, eps2, out_scale, ax, ay, az, splits ? split : nullptr, \
-        splits ? part : nullptr, splits ? nullptr : perm, splits ? nullptr : pcost, err, \
+        splits ? part : nullptr, perm, pcost, err, \
         counters
 #define NBH_LAUNCH(C, S, K)                                                      \
     do {                                                                         \

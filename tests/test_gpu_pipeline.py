@@ -165,14 +165,17 @@ def test_mixed_walk_independent_of_schedule(nb, n):
         s.close()
 
 
-@pytest.mark.parametrize("kind,n", [("clumps", 1_500_000), ("plummer", 2_097_152)])
+@pytest.mark.parametrize("kind,n", [("clumps", 1_500_000), ("plummer", 2_097_152),
+                                    ("plummer", 100_000), ("clumps", 50_000)])
 def test_mixed_walk_cost_ordered_queue_covers_every_packet(nb, kind, n):
-    """Whole walks with more packets than the device has slots (two per
-    resident wavefront: 16,384 on 256 CUs) hand out their packets in an
-    order built from each packet's visits in the previous traversal.  After
-    a few steps the last traversal ran in such an order; a new simulation of
+    """The queue hands out the traversal's units in an order built from each
+    unit's visits in the previous traversal: the cheapest packets last for
+    whole walks with more packets than the device has slots (two per
+    resident wavefront: 16,384 on 256 CUs; 1.5M and 2M particles), every
+    subtree unit by decreasing cost for split walks (100K and 50K).  After a
+    few steps the last traversal ran in such an order; a new simulation of
     the same state runs its first traversal in curve order.  Both walk the
-    same tree, so every target's acceleration agrees bit for bit: a packet
+    same tree, so every target's acceleration agrees bit for bit: a unit
     left out or walked twice by the ordered queue would leave a stale or
     different value."""
     p = make(nb, kind, n, seed=5)
