@@ -375,24 +375,17 @@ __device__ __forceinline__ double4 leaf_sum(double4 p0, const double4* __restric
     return make_double4(cx, cy, cz, tm);
 }
 
+// One internal node v of level L (s_tab: the curve tables in LDS).
 template <int BITS>
-__global__ __launch_bounds__(256) void level_up_kernel(
-    nbh_octree_view o, int L, int begin, int end, const int* __restrict__ order,
-    LevelArrays la, const double4* __restrict__ rec64, const double* __restrict__ box,
-    double theta, bool collapse, const uint8_t* __restrict__ tab)
+__device__ __forceinline__ void level_up_node(const nbh_octree_view& o, int L, int v,
+                                              const LevelArrays& la,
+                                              const double4* __restrict__ rec64,
+                                              const double* __restrict__ box, double theta,
+                                              bool collapse, const uint8_t* s_tab)
 {
     using Key = typename Layout<BITS>::Key;
-    __shared__ uint8_t s_tab[kHilbertTabBytes];
-    for (int t = static_cast<int>(threadIdx.x); t < kHilbertTabBytes; t += blockDim.x)
-        s_tab[t] = tab[t];
-    __syncthreads();
     const double rcx = box[0], rcy = box[1], rcz = box[2], side = box[3];
-    const int stride = static_cast<int>(gridDim.x * blockDim.x);
-    for (int k = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); k < end;
-         k += stride) {
-        const int v = order[k];
-        if (v < 0 || v >= o.n_nodes)
-            continue;
+    {
         const Key pv = static_cast<const Key*>(o.prefix)[v];
         const int4 c0 = reinterpret_cast<const int4*>(o.children)[2 * static_cast<size_t>(v)];
         const int4 c1 =
@@ -494,6 +487,55 @@ __global__ __launch_bounds__(256) void level_up_kernel(
     }
 }
 
+template <int BITS>
+__global__ __launch_bounds__(256) void level_up_kernel(
+    nbh_octree_view o, int L, int begin, int end, const int* __restrict__ order,
+    LevelArrays la, const double4* __restrict__ rec64, const double* __restrict__ box,
+    double theta, bool collapse, const uint8_t* __restrict__ tab)
+{
+    __shared__ uint8_t s_tab[kHilbertTabBytes];
+    for (int t = static_cast<int>(threadIdx.x); t < kHilbertTabBytes; t += blockDim.x)
+        s_tab[t] = tab[t];
+    __syncthreads();
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int k = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); k < end;
+         k += stride) {
+        const int v = order[k];
+        if (v < 0 || v >= o.n_nodes)
+            continue;
+        level_up_node<BITS>(o, L, v, la, rec64, box, theta, collapse, s_tab);
+    }
+}
+
+// Consecutive levels hi .. lo (hi >= lo) of at most kLevelRunNodes nodes
+// each, bottom-up in one block: a small level is bound by the latency of its
+// dependent loads, which a launch per level pays from HBM and this block
+// pays from L2 (the previous level's writes).  The block's barrier orders
+// the levels (all of a workgroup's wavefronts share one CU and its L1).
+constexpr int kLevelRunThreads = 512;
+constexpr int kLevelRunNodes = 512;
+
+template <int BITS>
+__global__ __launch_bounds__(kLevelRunThreads) void level_up_run_kernel(
+    nbh_octree_view o, int hi, int lo, LevelStarts st, const int* __restrict__ order,
+    LevelArrays la, const double4* __restrict__ rec64, const double* __restrict__ box,
+    double theta, bool collapse, const uint8_t* __restrict__ tab)
+{
+    __shared__ uint8_t s_tab[kHilbertTabBytes];
+    for (int t = static_cast<int>(threadIdx.x); t < kHilbertTabBytes; t += blockDim.x)
+        s_tab[t] = tab[t];
+    __syncthreads();
+    for (int L = hi; L >= lo; --L) {
+        for (int k = st.s[L] + static_cast<int>(threadIdx.x); k < st.s[L + 1];
+             k += kLevelRunThreads) {
+            const int v = order[k];
+            if (v >= 0 && v < o.n_nodes)
+                level_up_node<BITS>(o, L, v, la, rec64, box, theta, collapse, s_tab);
+        }
+        __syncthreads();
+    }
+}
+
 // Top-down over one level: pre-order position = parent's + own offset, the
 // curve offset made absolute, then the node's walk record.  The root
 // (v = 0, level 0) also writes the walk length and the terminal record at
@@ -503,19 +545,16 @@ __global__ __launch_bounds__(256) void level_up_kernel(
 // hold more than n_nodes records, and the position may still hold a record
 // of an earlier, longer walk.
 template <int BITS, bool MIXED>
-__global__ __launch_bounds__(256) void level_down_kernel(
-    nbh_octree_view o, int L, int begin, int end, const int* __restrict__ order,
-    LevelArrays la, const double4* __restrict__ rec64, const int4* __restrict__ rec32,
-    const double* __restrict__ box, bool need_center, double eps, WalkOut w,
-    int* __restrict__ wcount)
+__device__ __forceinline__ void level_down_node(const nbh_octree_view& o, int L, int v,
+                                                const LevelArrays& la,
+                                                const double4* __restrict__ rec64,
+                                                const int4* __restrict__ rec32,
+                                                const double* __restrict__ box,
+                                                bool need_center, double eps, const WalkOut& w,
+                                                int* __restrict__ wcount)
 {
     using Key = typename Layout<BITS>::Key;
-    const int stride = static_cast<int>(gridDim.x * blockDim.x);
-    for (int k = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); k < end;
-         k += stride) {
-        const int v = order[k];
-        if (v < 0 || v >= o.n_nodes)
-            continue;
+    {
         const int4 a = la.aux[v];
         int pre = 0, hoff = 0;
         if (v == 0) {
@@ -529,7 +568,7 @@ __global__ __launch_bounds__(256) void level_down_kernel(
         } else {
             const int parent = o.parent[v];
             if (parent < 0 || parent >= o.n_nodes)
-                continue;
+                return;
             const int2 lk = la.link[v];
             const int2 p = la.pos[parent];
             pre = p.x + lk.x;
@@ -537,7 +576,7 @@ __global__ __launch_bounds__(256) void level_down_kernel(
         }
         la.pos[v] = make_int2(pre, hoff);
         if (pre < 0 || pre >= o.n_nodes)
-            continue;
+            return;
         NodeVals nv;
         nv.mc = la.sum4[v];
         nv.geom = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -553,6 +592,45 @@ __global__ __launch_bounds__(256) void level_down_kernel(
                                 nv.geom.w, code);
         }
         emit_record<MIXED>(nv, pre, rec64, rec32, box, need_center, eps, w);
+    }
+}
+
+template <int BITS, bool MIXED>
+__global__ __launch_bounds__(256) void level_down_kernel(
+    nbh_octree_view o, int L, int begin, int end, const int* __restrict__ order,
+    LevelArrays la, const double4* __restrict__ rec64, const int4* __restrict__ rec32,
+    const double* __restrict__ box, bool need_center, double eps, WalkOut w,
+    int* __restrict__ wcount)
+{
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int k = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); k < end;
+         k += stride) {
+        const int v = order[k];
+        if (v < 0 || v >= o.n_nodes)
+            continue;
+        level_down_node<BITS, MIXED>(o, L, v, la, rec64, rec32, box, need_center, eps, w,
+                                     wcount);
+    }
+}
+
+// Consecutive small levels lo .. hi top-down in one block (as
+// level_up_run_kernel).
+template <int BITS, bool MIXED>
+__global__ __launch_bounds__(kLevelRunThreads) void level_down_run_kernel(
+    nbh_octree_view o, int lo, int hi, LevelStarts st, const int* __restrict__ order,
+    LevelArrays la, const double4* __restrict__ rec64, const int4* __restrict__ rec32,
+    const double* __restrict__ box, bool need_center, double eps, WalkOut w,
+    int* __restrict__ wcount)
+{
+    for (int L = lo; L <= hi; ++L) {
+        for (int k = st.s[L] + static_cast<int>(threadIdx.x); k < st.s[L + 1];
+             k += kLevelRunThreads) {
+            const int v = order[k];
+            if (v >= 0 && v < o.n_nodes)
+                level_down_node<BITS, MIXED>(o, L, v, la, rec64, rec32, box, need_center, eps,
+                                             w, wcount);
+        }
+        __syncthreads();
     }
 }
 
@@ -848,7 +926,7 @@ void launch_leaf_positions(hipStream_t s, const nbh_octree_view& o, const int* p
 // The pipeline's mass/COM + walk layout: the internal levels max_level..0
 // bottom-up (each leaf's sums are formed by its parent from its particles), then
 // 0..max_level top-down, then the leaves' records; one launch per level,
-// sized to its node count.
+// sized to its node count, or one block per run of small levels.
 template <int BITS>
 void launch_level_passes(hipStream_t s, const nbh_octree_view& o, const LevelStarts& st,
                          bool mixed, const double4* rec64, const int4* rec32, double theta,
@@ -873,12 +951,25 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, const LevelSta
         unsigned b = blocks_for(static_cast<size_t>(st.s[L + 1] - st.s[L]), 256);
         return b > 8192u ? 8192u : b;
     };
-    for (int L = max_level; L >= 0; --L) {
-        if (st.s[L + 1] == st.s[L])
+    // Runs of consecutive levels of at most kLevelRunNodes nodes each (the
+    // tree's top levels, and its deepest ones) take one block each; the
+    // other levels a launch each.
+    const auto small = [&](int L) { return st.s[L + 1] - st.s[L] <= kLevelRunNodes; };
+    for (int L = max_level; L >= 0;) {
+        if (small(L)) {
+            int lo = L;
+            while (lo > 0 && small(lo - 1))
+                --lo;
+            level_up_run_kernel<BITS><<<1, kLevelRunThreads, 0, s>>>(
+                o, L, lo, st, order, la, rec64, box, theta, collapse, htab);
+            NBH_HIP(hipGetLastError());
+            L = lo - 1;
             continue;
+        }
         level_up_kernel<BITS><<<grid(L), 256, 0, s>>>(o, L, st.s[L], st.s[L + 1], order, la,
                                                       rec64, box, theta, collapse, htab);
         NBH_HIP(hipGetLastError());
+        --L;
     }
     if (between)
         NBH_HIP(hipEventRecord(between, s));
@@ -891,9 +982,21 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, const LevelSta
     // {node, link} pairs: w32node is the array base and the links start one
     // int4 later, both indexed at 2 * position.
     const WalkOut w{w64, w64center, w32node, w32node ? w32node + 1 : nullptr, w32center, G};
-    for (int L = 0; L <= max_level; ++L) {
-        if (st.s[L + 1] == st.s[L])
+    for (int L = 0; L <= max_level;) {
+        if (small(L)) {
+            int hi = L;
+            while (hi < max_level && small(hi + 1))
+                ++hi;
+            if (mixed)
+                level_down_run_kernel<BITS, true><<<1, kLevelRunThreads, 0, s>>>(
+                    o, L, hi, st, order, la, rec64, rec32, box, need_center, eps, w, wcount);
+            else
+                level_down_run_kernel<BITS, false><<<1, kLevelRunThreads, 0, s>>>(
+                    o, L, hi, st, order, la, rec64, rec32, box, need_center, eps, w, wcount);
+            NBH_HIP(hipGetLastError());
+            L = hi + 1;
             continue;
+        }
         if (mixed)
             level_down_kernel<BITS, true><<<grid(L), 256, 0, s>>>(
                 o, L, st.s[L], st.s[L + 1], order, la, rec64, rec32, box, need_center, eps,
@@ -903,6 +1006,7 @@ void launch_level_passes(hipStream_t s, const nbh_octree_view& o, const LevelSta
                 o, L, st.s[L], st.s[L + 1], order, la, rec64, rec32, box, need_center, eps,
                 w, wcount);
         NBH_HIP(hipGetLastError());
+        ++L;
     }
     // Leaves last: a leaf that collapsed its single-child parent overwrites
     // the parent's record at the shared position.
