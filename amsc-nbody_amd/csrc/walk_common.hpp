@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kScheduleThreads) void packet_schedule_kernel(
 {
     __shared__ unsigned s_bin[kScheduleBins]; // histogram, then cursors
     __shared__ unsigned s_wave[kScheduleThreads / 64];
-    __shared__ unsigned s_cut[2];             // last tail bin, packets before the tail
+    __shared__ unsigned s_cut[2];             // last tail bin, units in the tail
     const int units = split ? packets * split[0] : packets;
     const int per = (units + 7) / 8;
     const int lo = static_cast<int>(blockIdx.x) * per;
@@ -151,28 +151,43 @@ __global__ __launch_bounds__(kScheduleThreads) void packet_schedule_kernel(
     for (int k = lo + tid; k < hi; k += kScheduleThreads)
         atomicAdd(&s_bin[bin_of(cost[k])], 1u);
     __syncthreads();
-    if (tid == 0) {
-        // the tail is bins 0..cut, the lowest that hold `tail` packets,
-        // laid out after the other packets from bin cut down to bin 0
-        const unsigned want = static_cast<unsigned>(min(tail, hi - lo));
-        unsigned below = 0;
-        int cut = 0;
-        while (cut < kScheduleBins - 1 && below + s_bin[cut] < want)
-            below += s_bin[cut++];
-        below += s_bin[cut];
-        const unsigned head = static_cast<unsigned>(hi - lo) - below;
-        unsigned at = head;
-        for (int b = cut; b >= 0; --b) {
-            const unsigned c = s_bin[b];
-            s_bin[b] = at;
-            at += c;
+    // The tail is bins 0..cut, the lowest that hold `tail` units, laid out
+    // after the other units from bin cut down to bin 0: inclusive prefix
+    // sums over the bins (one bin per thread of the first kScheduleBins),
+    // then each tail bin's cursor.
+    static_assert(kScheduleBins % 64 == 0 && kScheduleBins <= kScheduleThreads,
+                  "one bin per thread of whole wavefronts");
+    unsigned cnt = 0, incl = 0;
+    if (tid < kScheduleBins) {
+        cnt = s_bin[tid];
+        incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = static_cast<unsigned>(__shfl_up(static_cast<int>(incl), o));
+            incl += lane >= o ? y : 0u;
         }
-        s_cut[0] = static_cast<unsigned>(cut);
-        s_cut[1] = head;
+        if (lane == 63)
+            s_wave[wave] = incl;
+    }
+    __syncthreads();
+    if (tid < kScheduleBins) {
+        for (int w = 0; w < wave; ++w)
+            incl += s_wave[w];
+        const unsigned want = static_cast<unsigned>(min(tail, hi - lo));
+        const unsigned excl = incl - cnt;
+        // the first bin whose prefix reaches `want` (the last bin if none)
+        if ((incl >= want && excl < want) || (tid == kScheduleBins - 1 && incl < want)) {
+            s_cut[0] = static_cast<unsigned>(tid);
+            s_cut[1] = incl; // units in the tail
+        }
+    }
+    __syncthreads();
+    if (tid < kScheduleBins && tid <= static_cast<int>(s_cut[0])) {
+        const unsigned head = static_cast<unsigned>(hi - lo) - s_cut[1];
+        s_bin[tid] = head + (s_cut[1] - incl);
     }
     __syncthreads();
     const int cut = static_cast<int>(s_cut[0]);
-    unsigned base = 0; // head packets placed by earlier chunks
+    unsigned base = 0; // head units placed by earlier chunks
     for (int c0 = lo; c0 < hi; c0 += kScheduleThreads) {
         const int k = c0 + tid;
         const int b = k < hi ? bin_of(cost[k]) : -1;
