@@ -52,7 +52,7 @@ void launch_force_vjp_mixed(hipStream_t s, const nbh_octree_view& o, int max_lev
                             double gunit, double munit, bool contains, bool soft,
                             const double* abx, const double* aby, const double* abz,
                             double* scratch, double* gx, double* gy, double* gz,
-                            double* gm, int* err);
+                            double* gm, uint32_t* perm, const uint32_t* cost, int* err);
 size_t force_vjp_scratch_doubles(size_t n, size_t wcount, size_t n_nodes);
 
 namespace {
@@ -144,7 +144,7 @@ void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scr
             static_cast<int>(n), static_cast<float>(mixed_eps2(eps, scale)),
             G * s->rb->box[4] * scale * scale * scale, G * scale * scale, contains, eps > 0.0,
             abar, abar + n, abar + 2 * n, scratch.p, xbar, xbar + n, xbar + 2 * n, mbar,
-            s->errors + ERR_TRAVERSAL);
+            s->psched.p, s->psched.p + s->psched.n / 2, s->errors + ERR_TRAVERSAL);
     } else {
         launch_force_vjp_fp64(s->stream, o, s->cfg.bits_per_axis, s->pos.p,
                               s->w64.p, s->w64center.p, s->scalars.p + 3, static_cast<int>(M),

@@ -481,6 +481,8 @@ struct MixedVjp {
     size_t pstride;      // particle-source arrays: x, y, z and mass terms at
                          // pbx + {0, 1, 2, 3} pstride
     size_t nstride;      // node-source arrays likewise from nbx
+    const uint32_t* perm; // split walks: wavefront q walks unit perm[q] (units
+                          // numbered packet * subtrees + subtree), else null
 };
 
 constexpr int kSrcBatch = 4;
@@ -626,10 +628,23 @@ __global__ __launch_bounds__(kWalkThreads) void vjp_mixed_kernel(MixedVjp m, OpA
                                                                  int* err)
 {
     const int lane = threadIdx.x & 63;
-    const int unit = static_cast<int>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    const int packet = unit / m.units;
-    const int sub = unit - packet * m.units;
+    const int q = static_cast<int>(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int n = m.n;
+    int packet = q / m.units;
+    int sub = q - packet * m.units;
+    if (m.perm && m.split) {
+        // units in the order of their costs, dealt to the wavefronts in
+        // dispatch order (a permutation by construction; the bound keeps
+        // every access inside the units whatever it holds)
+        const int subtrees = __builtin_amdgcn_readfirstlane(m.split[0]);
+        const int total = (n + 63) / 64 * subtrees;
+        if (q >= total)
+            return;
+        const int u = __builtin_amdgcn_readfirstlane(
+            static_cast<int>(min(m.perm[q], static_cast<uint32_t>(total - 1))));
+        packet = u / subtrees;
+        sub = u - packet * subtrees;
+    }
     if (packet * 64 >= n)
         return;
     int begin = 0;
@@ -884,7 +899,7 @@ void launch_force_vjp_mixed(hipStream_t s, const nbh_octree_view& o, int max_lev
                             double gunit, double munit, bool contains, bool soft,
                             const double* abx, const double* aby, const double* abz,
                             double* scratch, double* gx, double* gy, double* gz,
-                            double* gm, int* err)
+                            double* gm, uint32_t* perm, const uint32_t* cost, int* err)
 {
     const size_t N = static_cast<size_t>(n);
     const size_t W = static_cast<size_t>(wcount_host);
@@ -915,12 +930,22 @@ void launch_force_vjp_mixed(hipStream_t s, const nbh_octree_view& o, int max_lev
     if (split)
         launch_walk_split(s, wrec, d_wcount, split);
     const int units = split ? kWalkSplitMax : 1;
+    // Split walks deal their units by decreasing cost: the force walk's
+    // visit counts per unit, which the VJP walk's visits equal (the same
+    // records, opening test and units).
+    const bool ordered = split && perm && cost && n > 0;
+    if (ordered) {
+        const int packets = (n + 63) / 64;
+        packet_schedule_kernel<<<8, kScheduleThreads, 0, s>>>(cost, packets, split,
+                                                             packets * kWalkSplitMax, perm);
+        NBH_HIP(hipGetLastError());
+    }
     // the batched source sums address component c at base + c * stride
     if (a.pby != a.pbx + N || a.pbz != a.pbx + 2 * N || a.pm != a.pbx + 3 * N ||
         a.nby != a.nbx + W || a.nbz != a.nbx + 2 * W || a.nm != a.nbx + 3 * W)
         fail(NBH_LOGIC_ERROR, "force VJP source arrays are not laid out at fixed strides");
     const MixedVjp mv{wrec, wcenter, d_wcount, split, rec32, idx, order, n, units, eps2,
-                      gunit, munit, N, W};
+                      gunit, munit, N, W, ordered ? perm : nullptr};
     const unsigned blocks = blocks_for((N + 63) / 64 * units, 4);
     if (contains) {
         if (soft)
