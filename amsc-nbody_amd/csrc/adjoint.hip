@@ -43,7 +43,8 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
                            double eps2, double G, bool contains,
                            const double* abx, const double* aby,
                            const double* abz, double* scratch, double* gx,
-                           double* gy, double* gz, double* gm, int* err);
+                           double* gy, double* gz, double* gm, int* split,
+                           uint32_t* perm, uint32_t* pcost, int* err);
 void launch_force_vjp_mixed(hipStream_t s, const nbh_octree_view& o, int max_level,
                             const int2* pos, const int4* aux, int4* wrec,
                             const int4* wcenter, const int* d_wcount, int* split,
@@ -151,7 +152,9 @@ void force_vjp(nbh_sim* s, const double* abar, double* xbar, DevBuf<double>& scr
                               s->rec64.p, s->idx.p, static_cast<int>(n), eps * eps,
                               s->cfg.gravitational_constant, contains, abar,
                               abar + n, abar + 2 * n, scratch.p, xbar, xbar + n,
-                              xbar + 2 * n, mbar, s->errors + ERR_TRAVERSAL);
+                              xbar + 2 * n, mbar, s->wsplit.p, s->psched.p,
+                              s->psched.p ? s->psched.p + s->psched.n / 2 : nullptr,
+                              s->errors + ERR_TRAVERSAL);
     }
     s->mark(PH_KICK);
     s->mark(PH_COUNT);

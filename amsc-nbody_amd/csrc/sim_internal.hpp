@@ -160,8 +160,9 @@ struct nbh_sim {
     nbh::DevBuf<int> scalars;     // [3] wcount ([0..2] unused)
     nbh::DevBuf<unsigned long long> counters;
     nbh::DevBuf<int> walkq;       // mixed traversal work-queue counters
-    // mixed traversal schedule: the queue order of its units, then each
-    // unit's cost in the last traversal (zeroed at creation)
+    // traversal schedule (mixed walks, and the FP64 force VJP of few
+    // packets): the queue order of the units, then each unit's cost in the
+    // last walk (zeroed at creation)
     nbh::DevBuf<uint32_t> psched;
     // mixed traversal split into subtree units (few packets only): the
     // units' position ranges and their partial sums

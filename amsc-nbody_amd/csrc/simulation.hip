@@ -66,6 +66,12 @@ void alloc_state(nbh_sim* s)
                                                   ? kWalkSplitMax : 1);
         s->psched.alloc(2 * units);
         NBH_HIP(hipMemsetAsync(s->psched.p + units, 0, units * sizeof(uint32_t), s->stream));
+    } else if (walk_mixed_splits(static_cast<int>(n))) {
+        // the FP64 force VJP splits its units likewise (the force walk does not)
+        const size_t units = (n + 63) / 64 * kWalkSplitMax;
+        s->wsplit.alloc(kWalkSplitInts);
+        s->psched.alloc(2 * units);
+        NBH_HIP(hipMemsetAsync(s->psched.p + units, 0, units * sizeof(uint32_t), s->stream));
     }
     s->gfirst.alloc(n);
     s->gcount.alloc(n);

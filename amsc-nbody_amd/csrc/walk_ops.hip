@@ -50,6 +50,13 @@ struct OpArgs {
     double* nbz;
     double* pm;                    // particle-source mass terms, Morton order
     double* nm;                    // node-source mass terms, by walk position
+    // FP64 VJP split into subtree units (few packets): the split table
+    // {s, then per subtree {first, end, -, -}}, the queue order (wavefront q
+    // walks unit perm[q]; units numbered packet * s + subtree) and each
+    // unit's visit count; null for whole-packet walks
+    const int* split;
+    const uint32_t* perm;
+    uint32_t* pcost;
 };
 
 
@@ -222,12 +229,32 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int packet = packet_of_block(nblocks_padded) * 4 + wave;
+    const bool split = OP == OP_VJP && a.split != nullptr;
+    int packet = packet_of_block(nblocks_padded) * 4 + wave;
+    int unit = 0;
+    int begin = 0;
+    int end = *d_wcount;
+    if (split) {
+        // units by decreasing cost, dealt to the wavefronts in dispatch
+        // order (a permutation by construction; the bound keeps every
+        // access inside the units whatever it holds)
+        const int q = static_cast<int>(blockIdx.x) * 4 + wave;
+        const int subtrees = __builtin_amdgcn_readfirstlane(a.split[0]);
+        const int total = (n + 63) / 64 * subtrees;
+        if (q >= total)
+            return;
+        unit = a.perm ? __builtin_amdgcn_readfirstlane(static_cast<int>(
+                            min(a.perm[q], static_cast<uint32_t>(total - 1))))
+                      : q;
+        packet = unit / subtrees;
+        const int sub = unit - packet * subtrees;
+        begin = __builtin_amdgcn_readfirstlane(a.split[1 + 4 * sub]);
+        end = __builtin_amdgcn_readfirstlane(a.split[2 + 4 * sub]);
+    }
     const int t = packet * 64 + lane;
     if (packet * 64 >= n)
         return;
     const bool valid = t < n;
-    const int wcount = *d_wcount;
     Lane<OP> L;
     __shared__ double4 s_slots64[OP == OP_VJP ? kWalkThreads / 64 : 1]
                                 [OP == OP_VJP ? kSrcBatch64 * 64 : 1];
@@ -257,10 +284,13 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
             L.bz = a.abz[o];
         }
     }
-    int next = valid ? 0 : 0x7FFFFFFF;
-    int i = 0;
-    while (i < wcount) {
+    int next = valid ? begin : 0x7FFFFFFF;
+    int i = begin;
+    unsigned visits = 0;
+    while (i < end) {
         i = __builtin_amdgcn_readfirstlane(i); // wave-uniform
+        if (split)
+            ++visits;
         const Walk64Rec r = load_record64(w64, i);
         const int skip = r.skip;
         const int kind = r.kind;
@@ -316,6 +346,8 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
     }
     if (OP == OP_VJP)
         L.flush_sources();
+    if (split && a.pcost && lane == 0)
+        a.pcost[unit] = visits;
     if (L.code != 0) {
         raise_error(err, L.code);
         return;
@@ -324,11 +356,50 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
         return;
     if (OP == OP_POTENTIAL) {
         a.phi[o] = L.phi;
+    } else if (split) {
+        // the units of a packet add their target terms (zeroed first)
+        atomicAdd(&a.tbx[t], L.gx);
+        atomicAdd(&a.tby[t], L.gy);
+        atomicAdd(&a.tbz[t], L.gz);
     } else {
         a.tbx[t] = L.gx;
         a.tby[t] = L.gy;
         a.tbz[t] = L.gz;
     }
+}
+
+// The split of the FP64 walk below its first record, which every target
+// opens (as for the mixed walk's split): the subtrees that follow position
+// 0, read off the skip links, as units {first, end}.  A first record that is
+// not an internal node with an opening test, or links that do not tile the
+// walk, give the single unit {0, wcount}.
+__global__ void walk_split64_kernel(const Walk64Rec* __restrict__ w64,
+                                    const int* __restrict__ d_wcount, int* __restrict__ split)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const int wc = *d_wcount;
+    bool ok = wc > 1 && w64[0].kind == KIND_INTERNAL && w64[0].thr != -INFINITY;
+    int k = 0;
+    int p = 1;
+    while (ok && p < wc) {
+        const int e = w64[p].skip;
+        if (k == kWalkSplitMax || e <= p || e > wc) {
+            ok = false;
+            break;
+        }
+        split[1 + 4 * k] = p;
+        split[2 + 4 * k] = e;
+        p = e;
+        ++k;
+    }
+    if (!ok || k == 0) {
+        split[0] = 1;
+        split[1] = 0;
+        split[2] = wc;
+        return;
+    }
+    split[0] = k;
 }
 
 // Top-down over one level: u_v = u_parent + nbar(pre_v) / M_v and
@@ -838,7 +909,8 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
                            double eps2, double G, bool contains,
                            const double* abx, const double* aby,
                            const double* abz, double* scratch, double* gx,
-                           double* gy, double* gz, double* gm, int* err)
+                           double* gy, double* gz, double* gm, int* split,
+                           uint32_t* perm, uint32_t* pcost, int* err)
 {
     // scratch: tb (3n) | pb (3n) | pm (n) | nb (3 wcount) | nm (wcount) |
     //          u (3 n_nodes) | w (n_nodes)
@@ -871,8 +943,30 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
     if (a.pbz != a.pbx + 2 * N || a.pm != a.pbx + 3 * N || a.nbz != a.nbx + 2 * W ||
         a.nm != a.nbx + 3 * W)
         fail(NBH_LOGIC_ERROR, "force VJP source arrays are not laid out at fixed strides");
-    launch_op<OP_VJP>(s, w64, wcenter, d_wcount, rec, idx, n, eps2, G, contains, a,
-                      err);
+    if (split && perm && pcost && n > 0) {
+        // few packets: units of one subtree below the first record each, by
+        // decreasing visit count in the previous product
+        const int packets = (n + 63) / 64;
+        walk_split64_kernel<<<1, 64, 0, s>>>(w64, d_wcount, split);
+        packet_schedule_kernel<<<8, kScheduleThreads, 0, s>>>(pcost, packets, split,
+                                                             packets * kWalkSplitMax, perm);
+        launch_fill_f64(s, tb, 0.0, 3 * N);
+        NBH_HIP(hipGetLastError());
+        a.split = split;
+        a.perm = perm;
+        a.pcost = pcost;
+        const unsigned blocks = blocks_for(static_cast<size_t>(packets) * kWalkSplitMax, 4);
+        if (contains)
+            walk_op_kernel<true, OP_VJP><<<blocks, kWalkThreads, 0, s>>>(
+                w64, wcenter, d_wcount, rec, idx, n, 0, eps2, G, a, err);
+        else
+            walk_op_kernel<false, OP_VJP><<<blocks, kWalkThreads, 0, s>>>(
+                w64, wcenter, d_wcount, rec, idx, n, 0, eps2, G, a, err);
+        NBH_HIP(hipGetLastError());
+    } else {
+        launch_op<OP_VJP>(s, w64, wcenter, d_wcount, rec, idx, n, eps2, G, contains, a,
+                          err);
+    }
     unsigned blocks = blocks_for(M, 256);
     if (blocks > 2048)
         blocks = 2048;
