@@ -317,7 +317,7 @@ void launch_walk_fp64(hipStream_t s, const Walk64Rec* w64, size_t wcap,
 // for walks of at most kWalkSplitPackets packets.  split: 1 + 4 *
 // kWalkSplitMax ints; part: kWalkSplitMax * 64 * 3 doubles per packet.
 constexpr int kWalkSplitMax = 8;
-constexpr int kWalkSplitPackets = 4096; // N <= 262,144
+constexpr int kWalkSplitPackets = 16384; // N <= 1,048,576
 constexpr int kWalkSplitInts = 1 + 4 * kWalkSplitMax;
 // Whether the mixed traversal of n targets is split (a function of n only,
 // so that results do not depend on the device).

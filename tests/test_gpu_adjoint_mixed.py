@@ -64,7 +64,8 @@ def test_mixed_force_vjp_theta0_close_to_direct(nb):
 
 @pytest.mark.parametrize("theta,eps,kind,n", [
     (0.5, 0.01, "plummer", 20000),    # split walk: marked subtree records
-    (0.5, 0.01, "plummer", 300000),   # whole walks
+    (0.5, 0.01, "plummer", 300000),   # split walk, 8 wavefronts per SIMD
+    (0.5, 0.01, "plummer", 1100000),  # whole walks
     (1.2, 0.01, "clumps", 20000),     # cell containment test
     (0.5, 0.0, "plummer", 20000),     # no softening: leaves walked exactly
 ])

@@ -137,13 +137,13 @@ def test_mixed_walk_single_particle_is_zero(nb):
         assert np.array_equal(a, np.zeros(1))
 
 
-@pytest.mark.parametrize("n", [50_000, 300_000])
+@pytest.mark.parametrize("n", [50_000, 300_000, 1_100_000])
 def test_mixed_walk_independent_of_schedule(nb, n):
     """Wavefronts take units of work from a queue in whatever order they run;
     each unit's sums depend only on its targets and positions, so two
     simulations of the same input, and the same state walked again, agree
-    bit for bit.  50K targets split each packet's walk into subtree units;
-    300K walk whole packets."""
+    bit for bit.  50K and 300K targets split each packet's walk into subtree
+    units; 1.1M walk whole packets."""
     p = make(nb, "clumps", n, seed=3)
     sims = [nb.Simulation(p, 1e-4, 0.5, 1e-3, gravitational_constant=1.0,
                           precision=nb.PRECISION_MIXED, bits_per_axis=21)
@@ -225,10 +225,12 @@ def test_mixed_pipeline_any_units(nb, orc, kind, pos_scale, mass, G, eps):
 
 
 @pytest.mark.parametrize("kind,n", [("plummer", 262_144), ("plummer", 262_208),
-                                    ("uniform", 300_000)])
+                                    ("uniform", 300_000), ("plummer", 1_048_576),
+                                    ("plummer", 1_048_640)])
 def test_mixed_split_and_whole_walks_match_fp64(nb, kind, n):
-    """Either side of the size where the walk stops being split into subtree
-    units (4096 packets): the mixed accelerations stay within the mixed
+    """Either side of the sizes where the split walk changes its occupancy
+    (4,096 packets) and where the walk stops being split into subtree units
+    (16,384 packets): the mixed accelerations stay within the mixed
     tolerance of the FP64 pipeline, which is bit-exact with the oracle."""
     p = make(nb, kind, n, seed=9)
     out = {}
