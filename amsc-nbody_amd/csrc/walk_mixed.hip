@@ -838,14 +838,17 @@ void launch_walk_mixed(hipStream_t s, int4* wpairs, size_t wcap,
         NBH_HIP(hipGetLastError());
     }
     // Whole walks: one pair of packets per wavefront, 8 resident per SIMD.
-    // Split walks: at least four subtree units per wavefront; up to 4,096
-    // packets at most 2 wavefronts per SIMD, whose short units run faster
-    // with few wavefronts (SI blob, cost-ordered units, step time: 250K
-    // particles 1.42 ms at 2 per SIMD against 1.50 at 3 and 1.58 at 4; 100K
-    // 1.21 ms, 30K 0.97 ms); beyond, 8 per SIMD (Plummer traversal, split
-    // against whole packets: 512K 1.41 against 1.77 ms, 1M 2.28 against
-    // 2.73 ms; 1M uniform and clumps equal).
-    const int nb = splits ? walk_persistent_blocks(packets * kWalkSplitMax, 4,
+    // Split walks: up to 4,096 packets at least four subtree units per
+    // wavefront and at most 2 wavefronts per SIMD, whose short units run
+    // faster with few wavefronts (SI blob, cost-ordered units, step time:
+    // 250K particles 1.42 ms at 2 per SIMD against 1.50 at 3 and 1.58 at 4;
+    // 100K 1.21 ms, 30K 0.97 ms); beyond, at least 16 units per wavefront
+    // and at most 8 per SIMD (Plummer traversal: 512K 1.41 ms at 4 per SIMD
+    // against 1.46 at 8, 1M 2.28 ms at 8 against 2.38 at 6 and 2.97 at 4;
+    // split against whole packets: 512K 1.41 against 1.77 ms, 1M 2.28
+    // against 2.73 ms; 1M uniform and clumps equal).
+    const int nb = splits ? walk_persistent_blocks(packets * kWalkSplitMax,
+                                                   packets > 4096 ? 16 : 4,
                                                    packets > 4096 ? 8 : 2)
                           : walk_persistent_blocks(packets, 2, kWalkMixedWavesPerSimd);
     if (nb == 0)
