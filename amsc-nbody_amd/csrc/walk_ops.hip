@@ -220,7 +220,7 @@ struct Lane {
     }
 };
 
-template <bool CONTAINS, int OP>
+template <bool CONTAINS, int OP, bool SPLIT = false>
 __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
     const Walk64Rec* __restrict__ w64, const double4* __restrict__ wcenter,
     const int* __restrict__ d_wcount, const double4* __restrict__ rec,
@@ -229,7 +229,8 @@ __global__ __launch_bounds__(kWalkThreads) void walk_op_kernel(
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const bool split = OP == OP_VJP && a.split != nullptr;
+    // SPLIT (the VJP with a split table): a unit per wavefront
+    constexpr bool split = OP == OP_VJP && SPLIT;
     int packet = packet_of_block(nblocks_padded) * 4 + wave;
     int unit = 0;
     int begin = 0;
@@ -957,10 +958,10 @@ void launch_force_vjp_fp64(hipStream_t s, const nbh_octree_view& o,
         a.pcost = pcost;
         const unsigned blocks = blocks_for(static_cast<size_t>(packets) * kWalkSplitMax, 4);
         if (contains)
-            walk_op_kernel<true, OP_VJP><<<blocks, kWalkThreads, 0, s>>>(
+            walk_op_kernel<true, OP_VJP, true><<<blocks, kWalkThreads, 0, s>>>(
                 w64, wcenter, d_wcount, rec, idx, n, 0, eps2, G, a, err);
         else
-            walk_op_kernel<false, OP_VJP><<<blocks, kWalkThreads, 0, s>>>(
+            walk_op_kernel<false, OP_VJP, true><<<blocks, kWalkThreads, 0, s>>>(
                 w64, wcenter, d_wcount, rec, idx, n, 0, eps2, G, a, err);
         NBH_HIP(hipGetLastError());
     } else {
