@@ -47,6 +47,17 @@ constexpr int tile_size()
 {
     return kSortThreads * SortTile<K>::items;
 }
+// Up to kSmallSortKeys keys the tiles hold 1024 keys (4 per thread), so
+// that a pass spreads over more workgroups: few 4096-key tiles leave the
+// pass the latency of one tile's look-back chain (SI blob step: 100K 1.170
+// -> 1.145 ms, 250K 1.37 -> 1.355 ms; at 1M the small tiles cost 0.11 ms).
+constexpr int kSmallSortItems = 4;
+constexpr size_t kSmallSortKeys = size_t(1) << 18;
+inline size_t sort_tile_keys(size_t n)
+{
+    return n <= kSmallSortKeys ? static_cast<size_t>(kSortThreads * kSmallSortItems)
+                               : static_cast<size_t>(tile_size<uint64_t>());
+}
 // The look-back status is sized from the uint64 tile count.
 static_assert(tile_size<uint32_t>() >= tile_size<uint64_t>(),
               "uint32 tiles must not outnumber uint64 tiles");
@@ -87,14 +98,13 @@ __global__ __launch_bounds__(256) void digit_offsets_kernel(uint32_t* hist)
     h[threadIdx.x] = ex;
 }
 
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(256) void onesweep_kernel(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin,
     K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int shift,
     const uint32_t* __restrict__ digit_base, uint32_t* status,
     uint32_t* ticket)
 {
-    constexpr int ITEMS = SortTile<K>::items;
     constexpr int TILE = kSortThreads * ITEMS;
     __shared__ K s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
@@ -228,17 +238,12 @@ __global__ __launch_bounds__(256) void onesweep_kernel(
 
 } // namespace
 
-template <typename K>
-size_t sort_tiles(size_t n)
-{
-    return (n + tile_size<K>() - 1) / tile_size<K>();
-}
-
 // Words of look-back status + tickets needed for `passes` passes over n keys
 // (uint32 and uint64 tiles hold the same number of keys).
 size_t sort_status_words(size_t n, int passes)
 {
-    const size_t tiles = (n + tile_size<uint64_t>() - 1) / tile_size<uint64_t>();
+    const size_t tk = sort_tile_keys(n);
+    const size_t tiles = (n + tk - 1) / tk;
     return static_cast<size_t>(passes) * (tiles * 256 + 64);
 }
 
@@ -266,8 +271,10 @@ void launch_radix_passes(hipStream_t s, K* keys, uint32_t* vals, K* keys_alt,
         fail(NBH_INVALID_ARGUMENT, "radix sort needs an even pass count");
     digit_offsets_kernel<<<passes, 256, 0, s>>>(hist);
     NBH_HIP(hipGetLastError());
-    const size_t tiles = sort_tiles<K>(n);
-    const size_t per_pass = (n + tile_size<uint64_t>() - 1) / tile_size<uint64_t>() * 256 + 64;
+    const size_t tk = sort_tile_keys(n);
+    const size_t tiles = (n + tk - 1) / tk;
+    const size_t per_pass = tiles * 256 + 64;
+    const bool small = tk != static_cast<size_t>(tile_size<uint64_t>());
     K* src_k = keys;
     uint32_t* src_v = vals;
     K* dst_k = keys_alt;
@@ -275,10 +282,16 @@ void launch_radix_passes(hipStream_t s, K* keys, uint32_t* vals, K* keys_alt,
     for (int p = 0; p < passes; ++p) {
         uint32_t* st = status + static_cast<size_t>(p) * per_pass;
         uint32_t* ticket = st + (per_pass - 64);
-        (void)tiles;
-        onesweep_kernel<K><<<static_cast<unsigned>(sort_tiles<K>(n)), kSortThreads, 0, s>>>(
-            src_k, src_v, dst_k, dst_v, static_cast<uint32_t>(n), 8 * p,
-            hist + p * 256, st, ticket);
+        if (small)
+            onesweep_kernel<K, kSmallSortItems>
+                <<<static_cast<unsigned>(tiles), kSortThreads, 0, s>>>(
+                    src_k, src_v, dst_k, dst_v, static_cast<uint32_t>(n), 8 * p,
+                    hist + p * 256, st, ticket);
+        else
+            onesweep_kernel<K, SortTile<K>::items>
+                <<<static_cast<unsigned>(tiles), kSortThreads, 0, s>>>(
+                    src_k, src_v, dst_k, dst_v, static_cast<uint32_t>(n), 8 * p,
+                    hist + p * 256, st, ticket);
         NBH_HIP(hipGetLastError());
         K* tk = src_k;
         src_k = dst_k;
